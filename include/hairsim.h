@@ -1,0 +1,228 @@
+/*
+ * hairsim.h - C ABI of the MI355X strand-physics pass.
+ *
+ * This library replaces the reference's Vulkan compute step: the one GLSL
+ * kernel src/shaders/compute.comp:124-303 together with the host code that
+ * binds its buffers and dispatches it (src/Renderer.cpp:2022-2077 records the
+ * grid fill + vkCmdDispatch, src/Renderer.cpp:2309-2319 submits it each frame).
+ *
+ * Everything is plain C: opaque handles, plain pointers and sizes, status
+ * codes.  No C++ exceptions cross this boundary and no HIP or torch type
+ * appears in a signature.  All entry points return 0 (HS_OK) on success and a
+ * negative hs_status on failure; hs_last_error() then returns a message.
+ *
+ * Threading: one context per host thread, or serialise calls on a context
+ * yourself.  Steps are stream-ordered and asynchronous; hs_sync() and the
+ * hs_read_* calls block.
+ */
+#ifndef HAIRSIM_H
+#define HAIRSIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes                                                              */
+/* ------------------------------------------------------------------------ */
+typedef enum hs_status {
+    HS_OK = 0,
+    HS_ERR_INVALID = -1,     /* bad argument (null pointer, size out of range) */
+    HS_ERR_NO_DEVICE = -2,   /* no HIP device / wrong device ordinal          */
+    HS_ERR_OUT_OF_MEMORY = -3,
+    HS_ERR_HIP = -4,         /* a HIP runtime call failed                      */
+    HS_ERR_NOT_FOUND = -5,   /* unknown groom / mesh id                        */
+    HS_ERR_STATE = -6,       /* call not valid in the current state            */
+    HS_ERR_BUDGET = -7       /* admission refused: HBM budget exhausted        */
+} hs_status;
+
+/* ------------------------------------------------------------------------ */
+/* Boundary types: byte-for-byte the reference's buffer formats              */
+/* ------------------------------------------------------------------------ */
+
+/* Reference `Time` UBO, src/Scene.h:16-19 (8 B). */
+typedef struct hs_time {
+    float deltaTime;
+    float totalTime;
+} hs_time;
+
+/* Reference `Collider`, src/Scene.h:23-39 (three column-major mat4, 192 B).
+ * Collider 0 is the sphere (compute.comp:160-169: only transform[3].xyz, the
+ * centre, is used); colliders 1.. are ellipsoids (compute.comp:170-179). */
+typedef struct hs_collider {
+    float transform[16];
+    float inv[16];
+    float invTrans[16];
+} hs_collider;
+
+/* Reference `GridCell`, src/Scene.h:42-49 / compute.comp:35-38 (16 B):
+ * fixed-point (x1e6) velocity sum and density sum of one voxel. */
+typedef struct hs_grid_cell {
+    int32_t velocity[3];
+    int32_t density;
+} hs_grid_cell;
+
+/* Reference `StrandDrawIndirect`, src/Strand.h:53-58 (16 B). */
+typedef struct hs_draw_indirect {
+    uint32_t vertexCount;
+    uint32_t instanceCount;
+    uint32_t firstVertex;
+    uint32_t firstInstance;
+} hs_draw_indirect;
+
+/* The strand state itself crosses the boundary in the reference `Strand`
+ * layout (src/Strand.h:11-16, compute.comp:44-48) with a run-time N:
+ *     float strand[12*N] = { vec4 curvePoints[N]; vec4 curveVels[N];
+ *                            vec4 correctionVecs[N]; }
+ * i.e. 48*N bytes per strand, strands back to back. */
+#define HS_STRAND_FLOATS(N) (12u * (uint32_t)(N))
+
+#define HS_MAX_COLLIDERS 8u
+#define HS_MAX_POINTS 256u
+
+/* ------------------------------------------------------------------------ */
+/* Groom (one strand buffer = one reference `Hair`)                          */
+/* ------------------------------------------------------------------------ */
+
+/* Keep the per-point correction vectors (compute.comp:201) in device state so
+ * hs_read_strands can return them.  Off by default: they are per-step
+ * temporaries of the shader (written at :201, consumed at :214) and storing
+ * them costs 12 B per point per step. */
+#define HS_GROOM_KEEP_CORRECTIONS 0x1u
+
+typedef struct hs_groom_desc {
+    uint32_t num_strands;     /* S >= 1                                       */
+    uint32_t num_points;      /* N in [2, HS_MAX_POINTS] (NUM_CURVE_POINTS)   */
+    /* Rest length of one segment.  When seg_length is NULL every strand uses
+     * strand_length / (N - 1.0f), the reference's `radius`
+     * (compute.comp:139-140, strandLength = 2.5).  Otherwise seg_length[s]
+     * is strand s's segment rest length. */
+    float strand_length;
+    const float* seg_length;  /* optional, S floats                           */
+    /* Optional per-strand wind noise: strand s feels an extra acceleration
+     * wind_noise[s] * noise_dir (no reference counterpart: its wind is
+     * commented out at compute.comp:151-152). */
+    const float* wind_noise;  /* optional, S floats                           */
+    float noise_dir[3];
+    /* Hair-hair voxel grid (compute.comp:9-11, 204-206): G^3 cells of side h
+     * from origin.  grid_dim == 0 disables the grid for this groom.  The
+     * reference uses G = 64, h = 7/64, origin (-3,-2,-5). */
+    uint32_t grid_dim;
+    float grid_origin[3];
+    float grid_h;
+    uint32_t flags;           /* HS_GROOM_*                                   */
+} hs_groom_desc;
+
+/* ------------------------------------------------------------------------ */
+/* Per-step dispatch parameters                                              */
+/* ------------------------------------------------------------------------ */
+
+/* Every constant compute.comp hard-codes becomes a field here; the values in
+ * brackets are the reference's. */
+typedef struct hs_step_params {
+    float gravity[3];        /* (0,-9.8,0)              compute.comp:150       */
+    float wind[3];           /* uniform wind acceleration, added to gravity    */
+    float damping;           /* DAMPING 0.998           compute.comp:7         */
+    float spring_k;          /* penalty stiffness 1900  compute.comp:163,173   */
+    float sphere_radius;     /* collider 0 radius 1.0   compute.comp:161       */
+    float vmax;              /* velocity clamp 10       compute.comp:198-200   */
+    float friction;          /* hair-hair friction 0.08 compute.comp:296       */
+    uint32_t num_colliders;  /* colliders used (NUM_COLLIDERS 6)               */
+    uint32_t hair_hair;      /* 1: splat + grid friction; 0: skip the grid     */
+    uint32_t mesh_id;        /* 0: none; else a mesh collider (hs_mesh_create) */
+} hs_step_params;
+
+/* Fill `p` with the reference's constants (DAMPING 0.998, k 1900, sphere
+ * radius 1.0, vmax 10, friction 0.08, gravity (0,-9.8,0), 6 colliders,
+ * hair-hair on, no wind, no mesh). */
+void hs_step_params_reference(hs_step_params* p);
+
+/* Fill `d` with the reference groom defaults for S strands of N points:
+ * strand length 2.5, grid 64^3 with h = 7/64 at (-3,-2,-5). */
+void hs_groom_desc_reference(hs_groom_desc* d, uint32_t num_strands, uint32_t num_points);
+
+/* ------------------------------------------------------------------------ */
+/* Context                                                                   */
+/* ------------------------------------------------------------------------ */
+typedef struct hs_ctx hs_ctx;
+
+/* Replaces the Vulkan instance/device/queue bring-up for the compute path
+ * (src/Instance.cpp:315, src/Device.cpp:19-21): bind HIP device `device` and
+ * create the context's stream. */
+int hs_create(hs_ctx** out, int device);
+void hs_destroy(hs_ctx* ctx);
+const char* hs_last_error(const hs_ctx* ctx);
+/* Block until all work queued on the context has finished. */
+int hs_sync(hs_ctx* ctx);
+/* Run the context's work on a caller-owned hipStream_t (passed as void*), or
+ * NULL to return to the context's own stream. */
+int hs_set_stream(hs_ctx* ctx, void* hip_stream);
+/* Device bytes currently held by the context (strand state, grids, meshes). */
+int hs_memory_used(const hs_ctx* ctx, uint64_t* bytes);
+
+/* ------------------------------------------------------------------------ */
+/* Grooms                                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces `Hair::Hair` uploading its strand SSBO and indirect-draw buffer
+ * (src/Strand.cpp:149-191, BufferUtils::CreateBufferFromData at :188-190).
+ * `strands_aos` holds S strands in the reference Strand layout (48*N B each);
+ * it is copied, the caller keeps ownership. */
+int hs_groom_create(hs_ctx* ctx, const hs_groom_desc* desc,
+                    const float* strands_aos, uint32_t* groom_id);
+/* Replaces `Hair::~Hair` (src/Strand.cpp:214-223). */
+int hs_groom_destroy(hs_ctx* ctx, uint32_t groom_id);
+/* Device bytes one groom of this shape needs (state + grids). */
+uint64_t hs_groom_bytes(uint32_t num_strands, uint32_t num_points,
+                        uint32_t grid_dim, uint32_t flags);
+
+/* One physics step of one groom: the per-frame `Renderer::Frame` compute
+ * submit (src/Renderer.cpp:2309-2319), i.e. grid clear (:2063) + dispatch of
+ * compute.comp (:2070) with the Time UBO (compute.comp:20-23) and Collider UBO
+ * (compute.comp:25-33) contents given here.  Asynchronous. */
+int hs_step(hs_ctx* ctx, uint32_t groom_id, const hs_time* time,
+            const hs_collider* colliders, uint32_t num_colliders,
+            const hs_step_params* params);
+
+/* Read the groom back in the reference Strand layout (48*N B per strand) with
+ * the velocities after this step's grid friction, plus the indirect draw
+ * record {S, 1, 0, 0} the shader's count at compute.comp:302 produces.
+ * Either output may be NULL.  Blocks. */
+int hs_read_strands(hs_ctx* ctx, uint32_t groom_id, float* strands_aos,
+                    hs_draw_indirect* indirect);
+/* Overwrite the groom's state from a reference-layout buffer (checkpoint
+ * restore; clears any pending grid friction). */
+int hs_write_strands(hs_ctx* ctx, uint32_t groom_id, const float* strands_aos);
+/* Read the voxel grid the last step splatted (compute.comp:217-252), G^3
+ * cells in GridCell layout, index a + b*G + c*G*G.  Blocks. */
+int hs_read_grid(hs_ctx* ctx, uint32_t groom_id, hs_grid_cell* cells);
+
+/* ------------------------------------------------------------------------ */
+/* Kernel timing (no reference counterpart: the reference has no timers)     */
+/* ------------------------------------------------------------------------ */
+
+/* When on, every step launch is bracketed by HIP events recorded on the
+ * stream the kernel runs on.  Enabling (or disabling) resets the totals. */
+int hs_profile_enable(hs_ctx* ctx, int on);
+/* Number of timed launches and their summed kernel time in ms.  Blocks. */
+int hs_profile_read(hs_ctx* ctx, uint64_t* launches, double* total_ms);
+
+/* ------------------------------------------------------------------------ */
+/* Host-side collider helpers (src/Scene.h:28-38, src/Scene.cpp:110-136)      */
+/* ------------------------------------------------------------------------ */
+
+/* Collider(trans, rotDeg, scale): transform = T * Rz * Ry * Rx * S,
+ * inv = inverse(transform), invTrans = transpose(inv). */
+void hs_collider_from_trs(hs_collider* out, const float translation[3],
+                          const float rotation_deg[3], const float scale[3]);
+/* Scene::translateSphere: transform = transform * translate(t); inv and
+ * invTrans recomputed. */
+void hs_collider_translate(hs_collider* c, const float translation[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HAIRSIM_H */

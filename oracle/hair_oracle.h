@@ -1,0 +1,98 @@
+/*
+ * hair_oracle.h - CPU oracle for the strand-physics pass (TEST INFRASTRUCTURE).
+ *
+ * This is the checker, never the product: only tests/, the smoke check and
+ * bench.py's CPU-baseline leg may load it.  It restates the reference's
+ * compute shader src/shaders/compute.comp:124-303 line by line on the
+ * reference's own AoS `Strand` layout (compute.comp:44-48), once in binary32
+ * (ho_step_f32, "ref32") and once in binary64 (ho_step_f64, "oracle64"), with
+ * the racy grid phases of the shader (workgroup-only barrier() at
+ * compute.comp:209 and :255) resolved as a true global split: every strand
+ * splats before any strand gathers.
+ *
+ * Arithmetic contract (both precisions; the HIP kernels follow the same one):
+ *  - every +,-,*,/ and sqrt is one correctly rounded IEEE operation, no fused
+ *    multiply-add (built with -ffp-contract=off), left-to-right evaluation as
+ *    the GLSL expression is written;
+ *  - dot(a,b) = (ax*bx + ay*by) + az*bz; length(v) = sqrt(dot(v,v));
+ *    distance(a,b) = length(a-b);
+ *  - normalize(v) = v * (1/length(v))  (one reciprocal, three products: the
+ *    usual v*rsq(dot) lowering with rsq taken as a rounded 1/sqrt);
+ *  - division by a dispatch-uniform scalar (deltaTime at compute.comp:195 and
+ *    :214, grid cell size h at :219-221 and :260-262) is a product with the
+ *    rounded reciprocal computed once (1/dt, 1/h); every other division is a
+ *    true division.  GLSL 4.50 allows 2.5 ulp for `/`, so both readings are
+ *    the shader's;
+ *  - mat4 * vec4 (column-major m[c*4+r]):
+ *      r_row = ((m0*x + m1*y) + m2*z) + m3*w;
+ *  - clamp(x,0,1) = min(max(x,0),1) with IEEE maxNum/minNum (NaN -> 0);
+ *  - int(f): truncation, saturating to [INT32_MIN, INT32_MAX], NaN -> 0
+ *    (the hardware cvt; the shader's out-of-range case is undefined);
+ *  - grid atomicAdd wraps modulo 2^32.
+ */
+#ifndef HAIR_ORACLE_H
+#define HAIR_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HO_MAX_COLLIDERS 8
+
+typedef struct ho_params {
+    float dt;                 /* Time.deltaTime (compute.comp:142)          */
+    float gravity[3];         /* compute.comp:150                           */
+    float wind[3];            /* uniform wind (added to gravity)            */
+    float damping;            /* DAMPING (compute.comp:7)                   */
+    float spring_k;           /* compute.comp:163,173                       */
+    float sphere_radius;      /* compute.comp:161                           */
+    float vmax;               /* compute.comp:198-200                       */
+    float friction;           /* compute.comp:296                           */
+    int hair_hair;            /* 0: skip splat and gather                   */
+    int grid_dim;             /* GRID_DIM (compute.comp:9)                  */
+    float grid_origin[3];     /* compute.comp:206                           */
+    float grid_h;             /* compute.comp:205                           */
+    int num_colliders;        /* NUM_COLLIDERS (compute.comp:8)             */
+    const float* colliders;   /* num_colliders x 48 floats: transform, inv,
+                                 invTrans, each column-major (Scene.h:23-39) */
+    const float* seg_length;  /* S floats: segment rest length per strand   */
+    const float* wind_noise;  /* S floats or NULL                           */
+    float noise_dir[3];
+} ho_params;
+
+/* One step over S strands of N points, in place on `strands` (reference AoS
+ * layout, 12*N values per strand).  When hair_hair is set, `grid` (G^3 cells
+ * x 4 int32: vx, vy, vz, density) is cleared, splatted by every strand, then
+ * gathered by every strand; on return it holds this step's splat, as the
+ * shader's grid SSBO does after the dispatch. */
+void ho_step_f32(const ho_params* p, float* strands, int S, int N, int32_t* grid);
+void ho_step_f64(const ho_params* p, double* strands, int S, int N, int32_t* grid);
+
+/* The splat of one point alone into `grid` (compute.comp:217-252), and the
+ * gathered grid velocity at one point (compute.comp:258-294); exposed for the
+ * brute-force tests. */
+void ho_splat_point_f32(const ho_params* p, const float pos[3], const float vel[3], int32_t* grid);
+void ho_gather_point_f32(const ho_params* p, const float pos[3], const int32_t* grid, float out[3]);
+void ho_splat_point_f64(const ho_params* p, const double pos[3], const double vel[3], int32_t* grid);
+void ho_gather_point_f64(const ho_params* p, const double pos[3], const int32_t* grid, double out[3]);
+
+/* Penalty force of the collider loop at one position (compute.comp:155-184):
+ * returns the number of colliders hit and writes the total force
+ * (gravity + wind + noise*dir + sum/count) into out. */
+int ho_force_f32(const ho_params* p, const float pos[3], float noise, float out[3]);
+int ho_force_f64(const ho_params* p, const double pos[3], double noise, double out[3]);
+
+/* Saturating float->int32 (the int() of the contract above). */
+int32_t ho_sat_i32_f32(float f);
+int32_t ho_sat_i32_f64(double f);
+
+/* Multithreaded FP32 step (OpenMP over strands, same results as ho_step_f32);
+ * used as the timed CPU baseline.  threads <= 0 uses the OpenMP default. */
+void ho_step_f32_mt(const ho_params* p, float* strands, int S, int N, int32_t* grid, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,295 @@
+/*
+ * hair_oracle_body.h - body of the oracle, included twice by hair_oracle.c
+ * with R = float (suffix _f32) and R = double (suffix _f64).
+ * TEST INFRASTRUCTURE: see hair_oracle.h for the arithmetic contract.
+ *
+ * Every function cites the lines of src/shaders/compute.comp it restates.
+ */
+
+#define CAT2(a, b) a##b
+#define CAT(a, b) CAT2(a, b)
+#define F(name) CAT(name, SFX)
+
+typedef struct { R x, y, z; } F(v3_t);
+#define V3 F(v3_t)
+
+static inline V3 F(v3)(R x, R y, R z) { V3 v; v.x = x; v.y = y; v.z = z; return v; }
+static inline V3 F(vadd)(V3 a, V3 b) { return F(v3)(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline V3 F(vsub)(V3 a, V3 b) { return F(v3)(a.x - b.x, a.y - b.y, a.z - b.z); }
+/* scalar * vector, scalar on the left as the GLSL source writes it */
+static inline V3 F(smul)(R s, V3 a) { return F(v3)(s * a.x, s * a.y, s * a.z); }
+/* vector * scalar */
+static inline V3 F(muls)(V3 a, R s) { return F(v3)(a.x * s, a.y * s, a.z * s); }
+static inline R F(dot)(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+static inline R F(len)(V3 a) { return SQRT(F(dot)(a, a)); }
+/* normalize(v) = v * (1/length(v)) */
+static inline V3 F(nrm)(V3 a) { R inv = (R)1 / F(len)(a); return F(muls)(a, inv); }
+
+/* mat4 (column-major, m[c*4+r]) * vec4(v, w) -> xyz */
+static inline V3 F(mat_vec)(const float* m, V3 v, R w)
+{
+    R o[3];
+    for (int r = 0; r < 3; ++r)
+        o[r] = (((R)m[0 * 4 + r] * v.x + (R)m[1 * 4 + r] * v.y) + (R)m[2 * 4 + r] * v.z) + (R)m[3 * 4 + r] * w;
+    return F(v3)(o[0], o[1], o[2]);
+}
+
+static inline R F(clamp01)(R x) { return FMIN(FMAX(x, (R)0), (R)1); }
+
+int32_t F(ho_sat_i32)(R f)
+{
+    if (f != f) return 0;
+    if (f >= (R)2147483648.0) return INT32_MAX;
+    if (f < (R)-2147483648.0) return INT32_MIN;
+    return (int32_t)f;
+}
+
+/* Integer range of the cells one grid coordinate touches:
+ * [max(int(floor q), 0), min(int(floor q) + 1, G - 1)] (compute.comp:224-229). */
+static inline void F(cell_range)(R q, int G, int64_t* lo, int64_t* hi)
+{
+    int64_t f = (int64_t)F(ho_sat_i32)(FLOOR(q));
+    *lo = f > 0 ? f : 0;
+    *hi = (f + 1) < (int64_t)(G - 1) ? (f + 1) : (int64_t)(G - 1);
+}
+
+/* Grid-space coordinate of a position: (p - origin) / h, with /h taken as *(1/h)
+ * (compute.comp:219-221, 260-262). */
+static inline V3 F(grid_coord)(const ho_params* p, V3 pos)
+{
+    R inv_h = (R)1 / (R)p->grid_h;
+    V3 o = F(v3)((R)p->grid_origin[0], (R)p->grid_origin[1], (R)p->grid_origin[2]);
+    return F(muls)(F(vsub)(pos, o), inv_h);
+}
+
+/* Trilinear splat of one point into the fixed-point grid, compute.comp:217-252. */
+static void F(splat)(const ho_params* p, V3 pos, V3 vel, int32_t* grid, int atomic_adds)
+{
+    const int G = p->grid_dim;
+    const R SCALE = (R)1000000.0;
+    V3 q = F(grid_coord)(p, pos);
+    int64_t x0, x1, y0, y1, z0, z1;
+    F(cell_range)(q.x, G, &x0, &x1);
+    F(cell_range)(q.y, G, &y0, &y1);
+    F(cell_range)(q.z, G, &z0, &z1);
+    for (int64_t a = x0; a <= x1; ++a)
+        for (int64_t b = y0; b <= y1; ++b)
+            for (int64_t c = z0; c <= z1; ++c) {
+                int64_t idx = a + b * G + c * (int64_t)G * G;
+                R wx = F(clamp01)((R)1 - FABS(q.x - (R)a));
+                R wy = F(clamp01)((R)1 - FABS(q.y - (R)b));
+                R wz = F(clamp01)((R)1 - FABS(q.z - (R)c));
+                R w = (wx * wy) * wz;
+                V3 wv = F(smul)(w, vel);
+                uint32_t add[4] = {(uint32_t)F(ho_sat_i32)(SCALE * wv.x),
+                                   (uint32_t)F(ho_sat_i32)(SCALE * wv.y),
+                                   (uint32_t)F(ho_sat_i32)(SCALE * wv.z),
+                                   (uint32_t)F(ho_sat_i32)(SCALE * w)};
+                uint32_t* cell = (uint32_t*)(grid + 4 * idx);
+                for (int k = 0; k < 4; ++k) {
+                    if (atomic_adds)
+                        __atomic_fetch_add(&cell[k], add[k], __ATOMIC_RELAXED);
+                    else
+                        cell[k] += add[k]; /* wraps mod 2^32 like atomicAdd */
+                }
+            }
+}
+
+/* Grid velocity at one point, compute.comp:258-294: sum over the touched cells
+ * with density > 0 of (w * (1/float(density))) * vec3(velocity). */
+static V3 F(gather)(const ho_params* p, V3 pos, const int32_t* grid)
+{
+    const int G = p->grid_dim;
+    V3 q = F(grid_coord)(p, pos);
+    int64_t x0, x1, y0, y1, z0, z1;
+    F(cell_range)(q.x, G, &x0, &x1);
+    F(cell_range)(q.y, G, &y0, &y1);
+    F(cell_range)(q.z, G, &z0, &z1);
+    V3 gv = F(v3)(0, 0, 0);
+    for (int64_t a = x0; a <= x1; ++a)
+        for (int64_t b = y0; b <= y1; ++b)
+            for (int64_t c = z0; c <= z1; ++c) {
+                const int32_t* cell = grid + 4 * (a + b * G + c * (int64_t)G * G);
+                int32_t d = cell[3];
+                if (d > 0) {
+                    R wx = F(clamp01)((R)1 - FABS(q.x - (R)a));
+                    R wy = F(clamp01)((R)1 - FABS(q.y - (R)b));
+                    R wz = F(clamp01)((R)1 - FABS(q.z - (R)c));
+                    R w = (wx * wy) * wz;
+                    R t = w * ((R)1 / (R)d);
+                    gv = F(vadd)(gv, F(smul)(t, F(v3)((R)cell[0], (R)cell[1], (R)cell[2])));
+                }
+            }
+    return gv;
+}
+
+/* Force at a start-of-step position: gravity (+ wind, + per-strand noise along
+ * noise_dir) then the collider penalty loop, compute.comp:150-184. */
+static int F(force)(const ho_params* p, V3 x, R noise, V3* out)
+{
+    V3 f = F(vadd)(F(v3)((R)p->gravity[0], (R)p->gravity[1], (R)p->gravity[2]),
+                   F(v3)((R)p->wind[0], (R)p->wind[1], (R)p->wind[2]));
+    if (p->wind_noise)
+        f = F(vadd)(f, F(smul)(noise, F(v3)((R)p->noise_dir[0], (R)p->noise_dir[1], (R)p->noise_dir[2])));
+    const R k = (R)p->spring_k;
+    V3 acc = F(v3)(0, 0, 0);
+    int hits = 0;
+    for (int j = 0; j < p->num_colliders; ++j) {
+        const float* T = p->colliders + 48 * j;
+        const float* Inv = T + 16;
+        const float* InvT = T + 32;
+        if (j == 0) {
+            /* sphere: compute.comp:160-169 */
+            V3 c = F(v3)((R)T[12], (R)T[13], (R)T[14]);
+            V3 dv = F(vsub)(x, c);
+            R dist = F(len)(dv);
+            R rad = (R)p->sphere_radius;
+            if (dist < rad) {
+                R d = rad - dist;
+                V3 n = F(muls)(dv, (R)1 / dist);
+                acc = F(vadd)(acc, F(smul)(k * d, n));
+                hits += 1;
+            }
+        } else {
+            /* ellipsoid: EllipsoidCollision / GetPointOnEllipsoid /
+             * GetEllipsoidNormal, compute.comp:64-79 and 170-179 */
+            V3 tp = F(mat_vec)(Inv, x, (R)1);
+            if (F(len)(tp) <= (R)1) {
+                V3 onunit = F(nrm)(tp);
+                V3 surf = F(mat_vec)(T, onunit, (R)1);
+                R d = F(len)(F(vsub)(surf, x));
+                V3 n = F(nrm)(F(nrm)(F(mat_vec)(InvT, tp, (R)0)));
+                acc = F(vadd)(acc, F(smul)(k * d, n));
+                hits += 1;
+            }
+        }
+    }
+    if (hits > 0) {
+        R m = (R)hits;
+        f = F(vadd)(f, F(v3)(acc.x / m, acc.y / m, acc.z / m));
+    }
+    *out = f;
+    return hits;
+}
+
+/* Phase 1 (compute.comp:144-202) then phase 2 (211-253) of one strand. */
+static void F(strand_phase12)(const ho_params* p, R* st, int N, int s, int32_t* grid, int atomic_adds)
+{
+    R* P = st;
+    R* Vl = st + 4 * N;
+    R* C = st + 8 * N;
+    const R dt = (R)p->dt;
+    const R inv_dt = (R)1 / dt;
+    const R dt2 = dt * dt;
+    const R D = (R)p->damping;
+    const R vmax = (R)p->vmax;
+    const R r = (R)p->seg_length[s];
+    const R noise = p->wind_noise ? (R)p->wind_noise[s] : (R)0;
+
+    for (int i = 1; i < N; ++i) {
+        V3 x = F(v3)(P[4 * i], P[4 * i + 1], P[4 * i + 2]);
+        V3 u = F(v3)(Vl[4 * i], Vl[4 * i + 1], Vl[4 * i + 2]);
+        V3 par = F(v3)(P[4 * (i - 1)], P[4 * (i - 1) + 1], P[4 * (i - 1) + 2]);
+        V3 f;
+        F(force)(p, x, noise, &f);
+        /* predictedPos = currentPos + dt*currentVel + dt*dt*force  (:187) */
+        V3 pred = F(vadd)(F(vadd)(x, F(smul)(dt, u)), F(smul)(dt2, f));
+        /* follow the leader (:191-192) */
+        V3 dir = F(nrm)(F(vsub)(pred, par));
+        V3 pn = F(vadd)(par, F(smul)(r, dir));
+        /* damped velocity + clamp + correction (:195-201) */
+        V3 vel = F(smul)(D, F(muls)(F(vsub)(pn, x), inv_dt));
+        if (F(len)(vel) > vmax)
+            vel = F(muls)(F(nrm)(vel), vmax);
+        V3 corr = F(smul)(D, F(vsub)(pn, pred));
+        P[4 * i] = pn.x; P[4 * i + 1] = pn.y; P[4 * i + 2] = pn.z; P[4 * i + 3] = (R)1;
+        Vl[4 * i] = vel.x; Vl[4 * i + 1] = vel.y; Vl[4 * i + 2] = vel.z; Vl[4 * i + 3] = (R)0;
+        C[4 * i] = corr.x; C[4 * i + 1] = corr.y; C[4 * i + 2] = corr.z; C[4 * i + 3] = (R)0;
+    }
+    for (int i = 1; i < N; ++i) {
+        /* DFTL velocity correction (:213-215) */
+        if (i != N - 1) {
+            V3 c1 = F(v3)(C[4 * (i + 1)], C[4 * (i + 1) + 1], C[4 * (i + 1) + 2]);
+            V3 v = F(v3)(Vl[4 * i], Vl[4 * i + 1], Vl[4 * i + 2]);
+            v = F(vsub)(v, F(muls)(c1, inv_dt));
+            Vl[4 * i] = v.x; Vl[4 * i + 1] = v.y; Vl[4 * i + 2] = v.z;
+        }
+        if (p->hair_hair) {
+            V3 pos = F(v3)(P[4 * i], P[4 * i + 1], P[4 * i + 2]);
+            V3 v = F(v3)(Vl[4 * i], Vl[4 * i + 1], Vl[4 * i + 2]);
+            F(splat)(p, pos, v, grid, atomic_adds);
+        }
+    }
+}
+
+/* Phase 3 (compute.comp:257-298) of one strand: friction blend with the grid. */
+static void F(strand_phase3)(const ho_params* p, R* st, int N, const int32_t* grid)
+{
+    R* P = st;
+    R* Vl = st + 4 * N;
+    const R mu = (R)p->friction;
+    const R omu = (R)1 - mu;
+    for (int i = 1; i < N; ++i) {
+        V3 pos = F(v3)(P[4 * i], P[4 * i + 1], P[4 * i + 2]);
+        V3 gv = F(gather)(p, pos, grid);
+        V3 v = F(v3)(Vl[4 * i], Vl[4 * i + 1], Vl[4 * i + 2]);
+        v = F(vadd)(F(smul)(omu, v), F(smul)(mu, gv));
+        Vl[4 * i] = v.x; Vl[4 * i + 1] = v.y; Vl[4 * i + 2] = v.z; Vl[4 * i + 3] = (R)0;
+    }
+}
+
+static void F(step_impl)(const ho_params* p, R* strands, int S, int N, int32_t* grid, int threads)
+{
+    const size_t stride = 12u * (size_t)N;
+    const int hh = p->hair_hair && p->grid_dim > 0;
+    ho_params q = *p;
+    q.hair_hair = hh;
+    if (hh) {
+        size_t cells = (size_t)p->grid_dim * p->grid_dim * p->grid_dim;
+        memset(grid, 0, cells * 4 * sizeof(int32_t)); /* vkCmdFillBuffer, Renderer.cpp:2063 */
+    }
+    if (threads == 1) {
+        for (int s = 0; s < S; ++s)
+            F(strand_phase12)(&q, strands + s * stride, N, s, grid, 0);
+        /* global barrier: the split the shader's workgroup barrier() misses */
+        if (hh)
+            for (int s = 0; s < S; ++s)
+                F(strand_phase3)(&q, strands + s * stride, N, grid);
+    } else {
+#ifdef _OPENMP
+        if (threads > 0) omp_set_num_threads(threads);
+#endif
+#pragma omp parallel for schedule(static)
+        for (int s = 0; s < S; ++s)
+            F(strand_phase12)(&q, strands + s * stride, N, s, grid, 1);
+        if (hh) {
+#pragma omp parallel for schedule(static)
+            for (int s = 0; s < S; ++s)
+                F(strand_phase3)(&q, strands + s * stride, N, grid);
+        }
+    }
+}
+
+void F(ho_splat_point)(const ho_params* p, const R pos[3], const R vel[3], int32_t* grid)
+{
+    F(splat)(p, F(v3)(pos[0], pos[1], pos[2]), F(v3)(vel[0], vel[1], vel[2]), grid, 0);
+}
+
+void F(ho_gather_point)(const ho_params* p, const R pos[3], const int32_t* grid, R out[3])
+{
+    V3 g = F(gather)(p, F(v3)(pos[0], pos[1], pos[2]), grid);
+    out[0] = g.x; out[1] = g.y; out[2] = g.z;
+}
+
+int F(ho_force)(const ho_params* p, const R pos[3], R noise, R out[3])
+{
+    V3 f;
+    int hits = F(force)(p, F(v3)(pos[0], pos[1], pos[2]), noise, &f);
+    out[0] = f.x; out[1] = f.y; out[2] = f.z;
+    return hits;
+}
+
+#undef V3
+#undef F
+#undef CAT
+#undef CAT2

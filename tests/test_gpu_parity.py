@@ -1,0 +1,177 @@
+"""GPU parity: the HIP step against the oracle's FP32 restatement (bit for bit)
+and its FP64 restatement (tolerance), through the C ABI.
+
+Tolerances (stated here, checked below):
+  * GPU vs ref32: bitwise equal strands and bitwise equal voxel ints.
+  * GPU vs oracle64, one step from the same FP32 state: positions within
+    1e-5 * max(|p|, 1); velocities within 1e-5 * max(|p|, 1) * damping / dt
+    (the position tolerance carried through v = D (p_new - x) / dt).
+"""
+import numpy as np
+import pytest
+
+import hairsim
+import oracle
+from hairsim import scenes
+from helpers import (assert_bitwise, gpu_params, make_gpu_groom, oracle_params, positions, run_parity,
+                     velocities)
+
+pytestmark = pytest.mark.gpu
+
+
+def test_plain_1k_240_steps_bitwise(ctx):
+    """1024 x 16, sphere, no hair-hair: every step bitwise equal to ref32 for 240 steps."""
+    wl = scenes.plain_1k()
+    run_parity(ctx, wl, 240, check_every=1)
+
+
+def test_plain_1k_keep_corrections_bitwise(ctx):
+    wl = scenes.plain_1k(S=256)
+    run_parity(ctx, wl, 12, keep_corrections=True)
+
+
+def test_hair_hair_grid_bitwise(ctx):
+    """Splat + global barrier + gather: strands and every voxel int bitwise."""
+    wl = scenes.plain_100k(S=2048, N=16, G=32)
+    run_parity(ctx, wl, 30, check_every=3)
+
+
+def test_plain_100k_shape_reduced_bitwise(ctx):
+    """The 100k x 32 workload's physics (wind, noise, orbiting sphere, 64^3 grid) on 8192 strands."""
+    wl = scenes.plain_100k(S=8192, N=32, G=64)
+    run_parity(ctx, wl, 8, check_every=2)
+
+
+@pytest.mark.parametrize("S,N", [(1, 2), (63, 3), (65, 10), (1000, 64), (129, 7)])
+def test_ragged_shapes_bitwise(ctx, S, N):
+    wl = scenes.plain_100k(S=S, N=N, G=16)
+    run_parity(ctx, wl, 5, check_every=1)
+
+
+def reference_demo_colliders():
+    """The six colliders of the reference demo (src/main.cpp:229-237)."""
+    spec = [((2.0, 0.0, 1.0), (0, 0, 0), (1, 1, 1)),
+            ((0.0, 2.64, 0.08), (-38.270, 0, 0), (0.817, 1.158, 1.01)),
+            ((0.0, 1.35, -0.288), (18.301, 0, 0), (0.457, 1.0, 0.538)),
+            ((0.0, -0.380, -0.116), (-17.260, 0, 0), (1.078, 1.683, 0.974)),
+            ((-0.698, 0.087, -0.36), (-20.254, 13.144, 34.5), (0.721, 1.0, 0.724)),
+            ((0.698, 0.087, -0.36), (-20.254, 13.144, -34.5), (0.721, 1.0, 0.724))]
+    return [hairsim.collider_from_trs(*s).as_array() for s in spec]
+
+
+def test_reference_constants_six_colliders_bitwise(ctx):
+    """Reference constants (DAMPING 0.998, friction 0.08, 64^3 grid at (-3,-2,-5), h 7/64, sphere r 1,
+    five ellipsoids) on strands hanging through the demo colliders."""
+    rng = np.random.default_rng(7)
+    S, N = 900, 10
+    roots, nrm = scenes.hemisphere_roots(rng, S, radius=1.0)
+    roots = roots + np.array([0.0, 2.3, 0.0])
+    aos, seg = scenes.straight_strands(roots, nrm, N, np.full(S, 2.5))
+    g = scenes.Groom(aos, N, seg, None, grid_dim=64, grid_origin=(-3.0, -2.0, -5.0), grid_h=7.0 / 64.0)
+    wl = scenes.Workload("reference-demo", dt=1.0 / 60.0, steps=20, damping=0.998, friction=0.08,
+                         hair_hair=True, sphere_radius=1.0, grooms=[g])
+    cols = reference_demo_colliders()
+    run_parity(ctx, wl, 20, check_every=2, ncol=6, colliders_fn=lambda k: cols)
+
+
+def test_vs_oracle64_per_step_tolerance(ctx):
+    """One step at a time from the GPU's FP32 state: GPU vs oracle64 within the stated tolerance."""
+    wl = scenes.plain_100k(S=2048, N=32, G=32)
+    g = wl.grooms[0]
+    gpu = make_gpu_groom(ctx, wl)
+    worst_p = worst_v = 0.0
+    try:
+        for k in range(30):
+            before, _ = gpu.read()
+            c = scenes.sphere_collider(wl.sphere(k))
+            gpu.step(wl.dt, float(wl.time(k)), [c], gpu_params(wl, k))
+            after, _ = gpu.read()
+            s64 = before.astype(np.float64)
+            oracle.step64(oracle_params(wl, k), s64, g.N)
+            p64, p32 = positions(s64, g.N), positions(after, g.N)
+            v64, v32 = velocities(s64, g.N), velocities(after, g.N)
+            scale = np.maximum(np.abs(p64), 1.0)
+            ep = np.max(np.abs(p32 - p64) / scale)
+            ev = np.max(np.abs(v32 - v64) / (scale * wl.damping / wl.dt))
+            worst_p, worst_v = max(worst_p, ep), max(worst_v, ev)
+    finally:
+        gpu.destroy()
+    assert worst_p <= 1e-5, worst_p
+    assert worst_v <= 1e-5, worst_v
+
+
+def test_full_100k_properties(ctx):
+    """Full 100,000 x 32 workload, 20 steps: finite, roots fixed, segment lengths, determinism."""
+    wl = scenes.plain_100k()
+    g = wl.grooms[0]
+    outs = []
+    for rep in range(2):
+        gpu = make_gpu_groom(ctx, wl)
+        for k in range(20):
+            gpu.step(wl.dt, float(wl.time(k)), [scenes.sphere_collider(wl.sphere(k))], gpu_params(wl, k))
+        out, ind = gpu.read()
+        grid = gpu.read_grid()
+        gpu.destroy()
+        outs.append((out, grid))
+        assert ind == (g.S, 1, 0, 0)
+    (a, ga), (b, gb) = outs
+    assert_bitwise(a, b, "run-to-run determinism", g.N)
+    assert np.array_equal(ga, gb)
+    P = positions(a, g.N)
+    assert np.isfinite(a).all()
+    assert np.array_equal(P[:, 0], positions(g.strands, g.N)[:, 0])
+    seg = np.linalg.norm(np.diff(P.astype(np.float64), axis=1), axis=2)
+    rel = np.abs(seg / g.seg[:, None].astype(np.float64) - 1.0)
+    # FP32 bound: endpoint rounding of |p| <= 4 against r >= 0.8/31
+    ulp = np.spacing(np.float32(4.0)).astype(np.float64)
+    assert rel.max() <= 8 * ulp / (0.8 / 31), rel.max()
+    # every point lands in the grid: total density ~ 1e6 per point
+    dens = ga.reshape(-1, 4)[:, 3].astype(np.int64).sum()
+    npts = g.S * (g.N - 1)
+    assert npts * (1e6 - 8) <= dens <= npts * 1e6
+
+
+def test_read_is_non_destructive_and_write_roundtrip(ctx):
+    wl = scenes.plain_100k(S=300, N=8, G=16)
+    g = wl.grooms[0]
+    gpu = make_gpu_groom(ctx, wl, keep_corrections=True)
+    try:
+        for k in range(3):
+            gpu.step(wl.dt, float(wl.time(k)), [scenes.sphere_collider(wl.sphere(k))], gpu_params(wl, k))
+        a, _ = gpu.read()
+        b, _ = gpu.read()
+        assert_bitwise(a, b, "second read")
+        gpu.write(a)
+        c, _ = gpu.read()
+        assert_bitwise(a, c, "write/read round trip")
+    finally:
+        gpu.destroy()
+
+
+def test_hair_hair_toggle_and_friction_zero(ctx):
+    """Grid off / on / off across steps, and friction 0 with the grid on equals the grid off."""
+    wl = scenes.plain_100k(S=512, N=16, G=16)
+    run_parity(ctx, wl, 6, params_kw=dict(hair_hair=False))
+    wl2 = scenes.plain_100k(S=512, N=16, G=16)
+    wl2.friction = 0.0
+    r_on, _ = run_parity(ctx, wl2, 6)
+    wl3 = scenes.plain_100k(S=512, N=16, G=16)
+    r_off, _ = run_parity(ctx, wl3, 6, params_kw=dict(hair_hair=False))
+    assert np.array_equal(positions(r_on, 16), positions(r_off, 16))
+    assert np.array_equal(velocities(r_on, 16), velocities(r_off, 16))
+
+
+def test_errors_are_status_codes(ctx):
+    wl = scenes.plain_1k(S=64)
+    gpu = make_gpu_groom(ctx, wl)
+    with pytest.raises(hairsim.HairSimError) as e:
+        gpu.step(0.0, 0.0, [], gpu_params(wl, 0, ncol=0))
+    assert e.value.code == hairsim.HS_ERR_INVALID
+    with pytest.raises(hairsim.HairSimError) as e:
+        gpu.step(wl.dt, 0.0, [], gpu_params(wl, 0, ncol=2))
+    assert e.value.code == hairsim.HS_ERR_INVALID
+    gpu.destroy()
+    gpu.id = 12345
+    with pytest.raises(hairsim.HairSimError) as e:
+        gpu.read()
+    assert e.value.code == hairsim.HS_ERR_NOT_FOUND
