@@ -207,8 +207,11 @@ int hs_read_grid(hs_ctx* ctx, uint32_t groom_id, hs_grid_cell* cells);
 /* When on, every step launch is bracketed by HIP events recorded on the
  * stream the kernel runs on.  Enabling (or disabling) resets the totals. */
 int hs_profile_enable(hs_ctx* ctx, int on);
-/* Number of timed launches and their summed kernel time in ms.  Blocks. */
+/* Number of timed steps and their summed kernel time in ms.  Blocks. */
 int hs_profile_read(hs_ctx* ctx, uint64_t* launches, double* total_ms);
+/* The same total split into the strand step kernel and the voxel reduce
+ * kernel (a step with hair-hair friction runs both).  Blocks. */
+int hs_profile_breakdown(hs_ctx* ctx, double* step_kernel_ms, double* reduce_kernel_ms);
 
 /* ------------------------------------------------------------------------ */
 /* Host-side collider helpers (src/Scene.h:28-38, src/Scene.cpp:110-136)      */

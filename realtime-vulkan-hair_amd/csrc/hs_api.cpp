@@ -11,8 +11,8 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
-#include <mutex>
 #include <new>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -21,23 +21,62 @@
 
 namespace {
 
-uint32_t round_up(uint32_t x, uint32_t m) { return (x + m - 1) / m * m; }
+uint64_t round_up(uint64_t x, uint64_t m) { return (x + m - 1) / m * m; }
+uint64_t grid_cells(uint32_t G) { return (uint64_t)G * G * G; }
+
+// Shape-derived sizes of one groom's device state.
+struct Layout {
+    uint32_t Sp, nwg, nseg;
+    uint64_t pool_cap, desc_cap;
+    uint64_t state_bytes, grid_bytes;
+};
+
+Layout layout_of(uint32_t S, uint32_t N, uint32_t G, uint32_t flags, bool noise)
+{
+    Layout l;
+    l.Sp = (uint32_t)round_up(S, hs::kW);
+    l.nwg = l.Sp / hs::kW;
+    l.nseg = (N - 1 + hs::kL - 1) / hs::kL;
+    const uint64_t planes = 3ull * N * l.Sp * 4;
+    l.state_bytes = 2 * planes + (uint64_t)l.Sp * 4 /*seg*/ + (uint64_t)l.Sp * 4 /*perm*/ +
+                    sizeof(hs::Collider) * hs::kMaxColliders;
+    if (noise) l.state_bytes += (uint64_t)l.Sp * 4;
+    if (flags & HS_GROOM_KEEP_CORRECTIONS) l.state_bytes += planes;
+    l.pool_cap = 0;
+    l.desc_cap = 0;
+    l.grid_bytes = 0;
+    if (G) {
+        // partial boxes: room for half a cell per point per step, at least 64k cells
+        l.pool_cap = std::max<uint64_t>(65536, (uint64_t)S * (N - 1) / 2);
+        l.desc_cap = 4ull * l.nwg * l.nseg + 4096;
+        l.grid_bytes = 3 * grid_cells(G) * 16                 // grid_i, grid_f, overflow
+                       + 2 * l.pool_cap * 16                  // pool halves
+                       + 2 * l.desc_cap * hs::kDescInts * 4   // descriptor halves
+                       + 8 * 4                                // counters
+                       + (uint64_t)l.nwg * l.nseg * 6 * 4;    // workgroup-segment boxes
+    }
+    return l;
+}
 
 struct Groom {
-    uint32_t S = 0, N = 0, Sp = 0, G = 0, flags = 0;
+    uint32_t S = 0, N = 0, G = 0, flags = 0;
+    Layout L{};
     float origin[3] = {0, 0, 0};
     float h = 1.0f;
+    float noise_dir[3] = {0, 0, 0};
     float* pos = nullptr;
     float* vel = nullptr;
     float* corr = nullptr;
     float* seg = nullptr;
     float* noise = nullptr;
-    float noise_dir[3] = {0, 0, 0};
-    int4* grid[3] = {nullptr, nullptr, nullptr};
-    bool dirty[3] = {false, false, false};
+    uint32_t* perm = nullptr;   // physical slot -> caller's strand index
+    hs::Collider* cols = nullptr;  // device copy of the step's colliders (ellipsoids)
+    hs::GridWork gw{};
+    std::vector<void*> allocs;  // every device allocation of this groom
     // Grid friction of the last step, not yet applied to the stored velocities.
-    int pending = -1;
+    bool pending = false;
     float pend_mu = 0.0f, pend_omu = 1.0f;
+    uint32_t parity = 0;        // pool/counter half the next splat writes
     uint64_t bytes = 0;
 };
 
@@ -51,12 +90,16 @@ struct hs_ctx {
     std::map<uint32_t, Groom> grooms;
     uint32_t next_id = 1;
     uint64_t bytes = 0;
-    // kernel timing (pooled events around each step launch on the context stream)
+    // kernel timing: pooled events (step start, step end, reduce end) per step
     bool profile = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    struct Ev {
+        hipEvent_t e[3];
+        bool reduce;
+    };
+    std::vector<Ev> events;
     size_t events_used = 0;
     uint64_t prof_launches = 0;
-    double prof_ms = 0.0;
+    double prof_step_ms = 0.0, prof_reduce_ms = 0.0;
 };
 
 namespace {
@@ -73,9 +116,9 @@ int hip_fail(hs_ctx* c, hipError_t e, const char* what)
     return fail(c, e == hipErrorOutOfMemory ? HS_ERR_OUT_OF_MEMORY : HS_ERR_HIP, m);
 }
 
-#define HS_TRY(ctx, expr)                                     \
-    do {                                                      \
-        hipError_t e_ = (expr);                               \
+#define HS_TRY(ctx, expr)                                      \
+    do {                                                       \
+        hipError_t e_ = (expr);                                \
         if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr); \
     } while (0)
 
@@ -83,12 +126,7 @@ void free_groom(hs_ctx* c, Groom& g)
 {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    (void)hipFree(g.pos);
-    (void)hipFree(g.vel);
-    (void)hipFree(g.corr);
-    (void)hipFree(g.seg);
-    (void)hipFree(g.noise);
-    for (int k = 0; k < 3; ++k) (void)hipFree(g.grid[k]);
+    for (void* p : g.allocs) (void)hipFree(p);
     c->bytes -= g.bytes;
     g = Groom();
 }
@@ -99,40 +137,97 @@ Groom* find(hs_ctx* c, uint32_t id)
     return it == c->grooms.end() ? nullptr : &it->second;
 }
 
-uint64_t grid_cells(uint32_t G) { return (uint64_t)G * G * G; }
-
 int drain_profile(hs_ctx* c)
 {
     for (size_t k = 0; k < c->events_used; ++k) {
         auto& ev = c->events[k];
-        HS_TRY(c, hipEventSynchronize(ev.second));
+        HS_TRY(c, hipEventSynchronize(ev.e[ev.reduce ? 2 : 1]));
         float ms = 0.0f;
-        HS_TRY(c, hipEventElapsedTime(&ms, ev.first, ev.second));
-        c->prof_ms += ms;
+        HS_TRY(c, hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
+        c->prof_step_ms += ms;
+        if (ev.reduce) {
+            HS_TRY(c, hipEventElapsedTime(&ms, ev.e[1], ev.e[2]));
+            c->prof_reduce_ms += ms;
+        }
         c->prof_launches += 1;
     }
     c->events_used = 0;
     return HS_OK;
 }
 
-// Next pair of timing events from the pool (created on first use).
-int next_events(hs_ctx* c, hipEvent_t* e0, hipEvent_t* e1)
+// Next set of timing events from the pool (created on first use).
+int next_events(hs_ctx* c, hs_ctx::Ev** out)
 {
     if (c->events_used == c->events.size()) {
         if (c->events.size() >= 4096) {
             int rc = drain_profile(c);
             if (rc != HS_OK) return rc;
         } else {
-            hipEvent_t a = nullptr, b = nullptr;
-            HS_TRY(c, hipEventCreate(&a));
-            HS_TRY(c, hipEventCreate(&b));
-            c->events.emplace_back(a, b);
+            hs_ctx::Ev ev{};
+            for (int k = 0; k < 3; ++k) HS_TRY(c, hipEventCreate(&ev.e[k]));
+            c->events.push_back(ev);
         }
     }
-    *e0 = c->events[c->events_used].first;
-    *e1 = c->events[c->events_used].second;
-    c->events_used += 1;
+    *out = &c->events[c->events_used++];
     return HS_OK;
+}
+
+// Largest float T with sqrtf(T) <= vmax, so that for every non-NaN d,
+// d > T exactly when sqrtf(d) > vmax (sqrtf is correctly rounded, hence
+// monotonic).  Lets the kernel test |v| > vmax without a square root.
+float sqrt_threshold(float vmax)
+{
+    if (std::isnan(vmax)) return vmax;                 // no d compares greater: never clamp
+    if (vmax < 0.0f) return -INFINITY;                 // sqrt(d) >= 0 > vmax for all d
+    if (std::isinf(vmax)) return INFINITY;
+    float x = vmax * vmax;
+    while (x > 0.0f && !(std::sqrt(x) <= vmax)) x = std::nextafter(x, 0.0f);
+    for (;;) {
+        const float nx = std::nextafter(x, INFINITY);
+        if (!std::isfinite(nx) || !(std::sqrt(nx) <= vmax)) break;
+        x = nx;
+    }
+    return x;
+}
+
+// Morton order of the roots: strands that start close are stored close, so a
+// workgroup's strands touch a compact set of voxels.
+std::vector<uint32_t> morton_perm(const float* aos, uint32_t S, uint32_t N)
+{
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t s = 0; s < S; ++s) {
+        const float* r = aos + (size_t)s * 12 * N;
+        for (int k = 0; k < 3; ++k)
+            if (std::isfinite(r[k])) {
+                lo[k] = std::min(lo[k], r[k]);
+                hi[k] = std::max(hi[k], r[k]);
+            }
+    }
+    auto spread = [](uint32_t v) {
+        v &= 0x3FF;
+        v = (v | (v << 16)) & 0x030000FF;
+        v = (v | (v << 8)) & 0x0300F00F;
+        v = (v | (v << 4)) & 0x030C30C3;
+        v = (v | (v << 2)) & 0x09249249;
+        return v;
+    };
+    std::vector<uint64_t> key(S);
+    for (uint32_t s = 0; s < S; ++s) {
+        const float* r = aos + (size_t)s * 12 * N;
+        uint32_t q[3];
+        for (int k = 0; k < 3; ++k) {
+            const float ext = hi[k] - lo[k];
+            float t = (ext > 0.0f && std::isfinite(r[k])) ? (r[k] - lo[k]) / ext : 0.0f;
+            t = std::min(std::max(t, 0.0f), 1.0f);
+            q[k] = (uint32_t)(t * 1023.0f);
+        }
+        const uint64_t code = spread(q[0]) | (spread(q[1]) << 1) | (spread(q[2]) << 2);
+        key[s] = (code << 32) | s;  // ties broken by index: deterministic
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<uint32_t> perm(S);
+    for (uint32_t p = 0; p < S; ++p) perm[p] = (uint32_t)(key[p] & 0xFFFFFFFFu);
+    return perm;
 }
 
 }  // namespace
@@ -195,10 +290,8 @@ void hs_destroy(hs_ctx* c)
     (void)hipStreamSynchronize(c->stream);
     for (auto& kv : c->grooms) free_groom(c, kv.second);
     c->grooms.clear();
-    for (auto& ev : c->events) {
-        (void)hipEventDestroy(ev.first);
-        (void)hipEventDestroy(ev.second);
-    }
+    for (auto& ev : c->events)
+        for (int k = 0; k < 3; ++k) (void)hipEventDestroy(ev.e[k]);
     if (c->own) (void)hipStreamDestroy(c->own);
     delete c;
 }
@@ -230,11 +323,9 @@ int hs_memory_used(const hs_ctx* c, uint64_t* bytes)
 
 uint64_t hs_groom_bytes(uint32_t S, uint32_t N, uint32_t G, uint32_t flags)
 {
-    const uint64_t Sp = round_up(S, 64);
-    uint64_t b = 2ull * 3 * N * Sp * 4 + 2ull * Sp * 4;
-    if (flags & HS_GROOM_KEEP_CORRECTIONS) b += 3ull * N * Sp * 4;
-    b += 3ull * grid_cells(G) * 16;
-    return b;
+    if (S == 0 || N < 2) return 0;
+    const Layout l = layout_of(S, N, G, flags, true);
+    return l.state_bytes + l.grid_bytes;
 }
 
 int hs_groom_create(hs_ctx* c, const hs_groom_desc* d, const float* aos, uint32_t* id)
@@ -245,8 +336,12 @@ int hs_groom_create(hs_ctx* c, const hs_groom_desc* d, const float* aos, uint32_
     if (d->num_points < 2 || d->num_points > HS_MAX_POINTS)
         return fail(c, HS_ERR_INVALID, "hs_groom_create: num_points must be in [2, 256]");
     if (d->num_strands > (1u << 26)) return fail(c, HS_ERR_INVALID, "hs_groom_create: too many strands");
+    // kernels index a component plane with 32-bit offsets: 3*N*Sp < 2^31
+    if ((uint64_t)d->num_points * round_up(d->num_strands, hs::kW) * 3 >= (1ull << 31))
+        return fail(c, HS_ERR_INVALID, "hs_groom_create: more than 715M points in one groom");
     if (d->grid_dim > 1024) return fail(c, HS_ERR_INVALID, "hs_groom_create: grid_dim > 1024");
-    if (d->grid_dim > 0 && !(d->grid_h > 0.0f)) return fail(c, HS_ERR_INVALID, "hs_groom_create: grid_h must be > 0");
+    if (d->grid_dim > 0 && !(d->grid_h > 0.0f && std::isfinite(d->grid_h)))
+        return fail(c, HS_ERR_INVALID, "hs_groom_create: grid_h must be finite and > 0");
     if (!d->seg_length && !(d->strand_length > 0.0f))
         return fail(c, HS_ERR_INVALID, "hs_groom_create: strand_length must be > 0");
     HS_TRY(c, hipSetDevice(c->device));
@@ -254,7 +349,6 @@ int hs_groom_create(hs_ctx* c, const hs_groom_desc* d, const float* aos, uint32_
     Groom g;
     g.S = d->num_strands;
     g.N = d->num_points;
-    g.Sp = round_up(g.S, 64);
     g.G = d->grid_dim;
     g.flags = d->flags;
     g.h = d->grid_h;
@@ -262,73 +356,81 @@ int hs_groom_create(hs_ctx* c, const hs_groom_desc* d, const float* aos, uint32_
         g.origin[k] = d->grid_origin[k];
         g.noise_dir[k] = d->noise_dir[k];
     }
-    const size_t planes = 3ull * g.N * g.Sp;
+    g.L = layout_of(g.S, g.N, g.G, g.flags, d->wind_noise != nullptr);
+    const Layout& L = g.L;
+    const size_t planes = 3ull * g.N * L.Sp;
     const size_t aos_bytes = (size_t)g.S * g.N * 48;
     float* staging = nullptr;
     auto cleanup = [&](int code, const std::string& msg) {
+        (void)hipStreamSynchronize(c->stream);
         (void)hipFree(staging);
-        (void)hipFree(g.pos);
-        (void)hipFree(g.vel);
-        (void)hipFree(g.corr);
-        (void)hipFree(g.seg);
-        (void)hipFree(g.noise);
-        for (int k = 0; k < 3; ++k) (void)hipFree(g.grid[k]);
+        for (void* p : g.allocs) (void)hipFree(p);
         return fail(c, code, msg);
     };
-#define HS_ALLOC(ptr, bytes)                                                          \
-    do {                                                                              \
-        hipError_t e_ = hipMalloc((void**)&(ptr), (bytes));                           \
-        if (e_ != hipSuccess)                                                         \
-            return cleanup(e_ == hipErrorOutOfMemory ? HS_ERR_OUT_OF_MEMORY : HS_ERR_HIP, \
-                           std::string("hipMalloc: ") + hipGetErrorString(e_));       \
+#define HS_ALLOC(ptr, nbytes)                                                                     \
+    do {                                                                                          \
+        void* p_ = nullptr;                                                                       \
+        hipError_t e_ = hipMalloc(&p_, (nbytes));                                                 \
+        if (e_ != hipSuccess)                                                                     \
+            return cleanup(e_ == hipErrorOutOfMemory ? HS_ERR_OUT_OF_MEMORY : HS_ERR_HIP,          \
+                           std::string("hipMalloc: ") + hipGetErrorString(e_));                   \
+        g.allocs.push_back(p_);                                                                   \
+        (ptr) = reinterpret_cast<decltype(ptr)>(p_);                                              \
+        e_ = hipMemsetAsync(p_, 0, (nbytes), c->stream);                                          \
+        if (e_ != hipSuccess) return cleanup(HS_ERR_HIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e_)); \
     } while (0)
-#define HS_CHECK(expr)                                                                \
-    do {                                                                              \
-        hipError_t e_ = (expr);                                                       \
+#define HS_CHECK(expr)                                                                              \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
         if (e_ != hipSuccess) return cleanup(HS_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
     } while (0)
 
     HS_ALLOC(g.pos, planes * 4);
     HS_ALLOC(g.vel, planes * 4);
     if (g.flags & HS_GROOM_KEEP_CORRECTIONS) HS_ALLOC(g.corr, planes * 4);
-    HS_ALLOC(g.seg, (size_t)g.Sp * 4);
-    if (d->wind_noise) HS_ALLOC(g.noise, (size_t)g.Sp * 4);
-    for (int k = 0; k < 3 && g.G; ++k) HS_ALLOC(g.grid[k], grid_cells(g.G) * 16);
-    HS_ALLOC(staging, aos_bytes);
-
-    // Padding lanes (s >= S) are never stepped; keep them zero.
-    HS_CHECK(hipMemsetAsync(g.pos, 0, planes * 4, c->stream));
-    HS_CHECK(hipMemsetAsync(g.vel, 0, planes * 4, c->stream));
-    if (g.corr) HS_CHECK(hipMemsetAsync(g.corr, 0, planes * 4, c->stream));
-    for (int k = 0; k < 3 && g.G; ++k) HS_CHECK(hipMemsetAsync(g.grid[k], 0, grid_cells(g.G) * 16, c->stream));
-
-    std::vector<float> seg(g.Sp, 0.0f);
-    if (d->seg_length) {
-        for (uint32_t s = 0; s < g.S; ++s) seg[s] = d->seg_length[s];
-    } else {
-        // radius = strandLength / (NUM_CURVE_POINTS - 1.0)  (compute.comp:139-140)
-        const float r = d->strand_length / ((float)g.N - 1.0f);
-        for (uint32_t s = 0; s < g.S; ++s) seg[s] = r;
+    HS_ALLOC(g.seg, (size_t)L.Sp * 4);
+    HS_ALLOC(g.perm, (size_t)L.Sp * 4);
+    HS_ALLOC(g.cols, sizeof(hs::Collider) * hs::kMaxColliders);
+    if (d->wind_noise) HS_ALLOC(g.noise, (size_t)L.Sp * 4);
+    if (g.G) {
+        const size_t cells = grid_cells(g.G);
+        HS_ALLOC(g.gw.grid_i, cells * 16);
+        HS_ALLOC(g.gw.grid_f, cells * 16);
+        HS_ALLOC(g.gw.overflow, cells * 16);
+        HS_ALLOC(g.gw.pool, 2 * L.pool_cap * 16);
+        HS_ALLOC(g.gw.desc, 2 * L.desc_cap * hs::kDescInts * 4);
+        HS_ALLOC(g.gw.counters, 8 * 4);
+        HS_ALLOC(g.gw.gbox, (size_t)L.nwg * L.nseg * 6 * 4);
+        g.gw.pool_cap = (uint32_t)L.pool_cap;
+        g.gw.desc_cap = (uint32_t)L.desc_cap;
     }
-    HS_CHECK(hipMemcpyAsync(g.seg, seg.data(), (size_t)g.Sp * 4, hipMemcpyHostToDevice, c->stream));
+    HS_CHECK(hipMalloc((void**)&staging, aos_bytes));
+
+    // physical order: Morton order of the roots
+    const std::vector<uint32_t> perm = morton_perm(aos, g.S, g.N);
+    std::vector<float> seg(L.Sp, 0.0f), nz;
+    // radius = strandLength / (NUM_CURVE_POINTS - 1.0)  (compute.comp:139-140)
+    const float r_uniform = d->seg_length ? 0.0f : d->strand_length / ((float)g.N - 1.0f);
+    for (uint32_t p = 0; p < g.S; ++p) seg[p] = d->seg_length ? d->seg_length[perm[p]] : r_uniform;
     if (d->wind_noise) {
-        std::vector<float> nz(g.Sp, 0.0f);
-        std::copy(d->wind_noise, d->wind_noise + g.S, nz.begin());
-        HS_CHECK(hipMemcpyAsync(g.noise, nz.data(), (size_t)g.Sp * 4, hipMemcpyHostToDevice, c->stream));
-        HS_CHECK(hipStreamSynchronize(c->stream));
+        nz.assign(L.Sp, 0.0f);
+        for (uint32_t p = 0; p < g.S; ++p) nz[p] = d->wind_noise[perm[p]];
     }
+    HS_CHECK(hipMemcpyAsync(g.seg, seg.data(), (size_t)L.Sp * 4, hipMemcpyHostToDevice, c->stream));
+    HS_CHECK(hipMemcpyAsync(g.perm, perm.data(), (size_t)g.S * 4, hipMemcpyHostToDevice, c->stream));
+    if (d->wind_noise) HS_CHECK(hipMemcpyAsync(g.noise, nz.data(), (size_t)L.Sp * 4, hipMemcpyHostToDevice, c->stream));
     HS_CHECK(hipMemcpyAsync(staging, aos, aos_bytes, hipMemcpyHostToDevice, c->stream));
-    HS_CHECK(hs::launch_pack(staging, g.S, g.N, g.Sp, g.pos, g.vel, g.corr, c->stream));
+    HS_CHECK(hs::launch_pack(staging, g.perm, g.S, g.N, L.Sp, g.pos, g.vel, g.corr, c->stream));
     HS_CHECK(hipStreamSynchronize(c->stream));
     HS_CHECK(hipFree(staging));
     staging = nullptr;
 #undef HS_ALLOC
 #undef HS_CHECK
 
-    g.bytes = hs_groom_bytes(g.S, g.N, g.G, g.flags) - (g.noise ? 0 : (uint64_t)g.Sp * 4);
+    g.bytes = L.state_bytes + L.grid_bytes;
     c->bytes += g.bytes;
     const uint32_t gid = c->next_id++;
-    c->grooms[gid] = g;
+    c->grooms[gid] = std::move(g);
     *id = gid;
     return HS_OK;
 }
@@ -367,8 +469,12 @@ int hs_step(hs_ctx* c, uint32_t id, const hs_time* t, const hs_collider* cols, u
     a.noise = g->noise;
     a.S = g->S;
     a.N = g->N;
-    a.Sp = g->Sp;
+    a.Sp = g->L.Sp;
+    a.nwg = g->L.nwg;
+    a.nseg = g->L.nseg;
+    a.gw = g->gw;
     a.G = g->G;
+    a.parity = g->parity;
     a.ox = g->origin[0];
     a.oy = g->origin[1];
     a.oz = g->origin[2];
@@ -379,6 +485,7 @@ int hs_step(hs_ctx* c, uint32_t id, const hs_time* t, const hs_collider* cols, u
     a.D = p->damping;
     a.k = p->spring_k;
     a.vmax = p->vmax;
+    a.vmax2 = sqrt_threshold(p->vmax);
     a.sphere_r = p->sphere_radius;
     a.fx = p->gravity[0] + p->wind[0];
     a.fy = p->gravity[1] + p->wind[1];
@@ -387,56 +494,48 @@ int hs_step(hs_ctx* c, uint32_t id, const hs_time* t, const hs_collider* cols, u
     a.ndy = g->noise_dir[1];
     a.ndz = g->noise_dir[2];
     a.ncol = p->num_colliders;
-    for (uint32_t j = 0; j < a.ncol; ++j) {
-        std::memcpy(a.col[j].m, cols[j].transform, 64);
-        std::memcpy(a.col[j].m + 16, cols[j].inv, 64);
-        std::memcpy(a.col[j].m + 32, cols[j].invTrans, 64);
+    a.cols = g->cols;
+    if (a.ncol > 0) {
+        a.sphere_c[0] = cols[0].transform[12];
+        a.sphere_c[1] = cols[0].transform[13];
+        a.sphere_c[2] = cols[0].transform[14];
     }
-
-    const bool gather = g->pending >= 0;
+    if (a.ncol > 1) {
+        // ellipsoids are read from device memory; stream order keeps the
+        // previous step's kernel ahead of this copy
+        hs::Collider host[hs::kMaxColliders];
+        for (uint32_t j = 0; j < a.ncol; ++j) {
+            std::memcpy(host[j].m, cols[j].transform, 64);
+            std::memcpy(host[j].m + 16, cols[j].inv, 64);
+            std::memcpy(host[j].m + 32, cols[j].invTrans, 64);
+        }
+        HS_TRY(c, hipMemcpyAsync(g->cols, host, sizeof(hs::Collider) * a.ncol, hipMemcpyHostToDevice, c->stream));
+    }
+    const bool gather = g->pending;
     if (gather) {
-        a.gprev = g->grid[g->pending];
         a.mu_prev = g->pend_mu;
         a.omu_prev = g->pend_omu;
     }
     const bool splat = p->hair_hair && g->G > 0;
-    int next = -1;
-    if (splat) {
-        // the splat target is any buffer that is not pending; it must be zero
-        next = (g->pending + 1 + 3) % 3;
-        if (g->dirty[next]) {
-            HS_TRY(c, hipMemsetAsync(g->grid[next], 0, grid_cells(g->G) * 16, c->stream));
-            g->dirty[next] = false;
-        }
-        // a buffer that is neither pending nor the target is cleared in-kernel
-        // for the following step
-        int clr = -1;
-        for (int k = 0; k < 3; ++k)
-            if (k != next && k != g->pending) clr = k;
-        if (clr >= 0 && g->dirty[clr]) {
-            a.gclear = g->grid[clr];
-            a.clear_cells = (uint32_t)grid_cells(g->G);
-            g->dirty[clr] = false;
-        }
-        a.gnext = g->grid[next];
-        g->dirty[next] = true;
-    }
 
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hs_ctx::Ev* ev = nullptr;
     if (c->profile) {
-        int rc = next_events(c, &e0, &e1);
+        int rc = next_events(c, &ev);
         if (rc != HS_OK) return rc;
-        HS_TRY(c, hipEventRecord(e0, c->stream));
+        ev->reduce = splat;
+        HS_TRY(c, hipEventRecord(ev->e[0], c->stream));
     }
     HS_TRY(c, hs::launch_step(a, gather, splat, c->stream));
-    if (c->profile) HS_TRY(c, hipEventRecord(e1, c->stream));
-
+    if (ev) HS_TRY(c, hipEventRecord(ev->e[1], c->stream));
     if (splat) {
-        g->pending = next;
+        HS_TRY(c, hs::launch_reduce(g->gw, g->G, g->parity, c->stream));
+        if (ev) HS_TRY(c, hipEventRecord(ev->e[2], c->stream));
+        g->parity ^= 1u;
+        g->pending = true;
         g->pend_mu = p->friction;
         g->pend_omu = 1.0f - p->friction;
     } else {
-        g->pending = -1;
+        g->pending = false;
     }
     return HS_OK;
 }
@@ -460,8 +559,8 @@ int hs_read_strands(hs_ctx* c, uint32_t id, float* aos, hs_draw_indirect* ind)
     HS_TRY(c, hipMalloc((void**)&dev, bytes));
     hs::UnpackGrid ug;
     const hs::UnpackGrid* pug = nullptr;
-    if (g->pending >= 0) {
-        ug.grid = g->grid[g->pending];
+    if (g->pending) {
+        ug.grid = g->gw.grid_i;
         ug.G = g->G;
         ug.ox = g->origin[0];
         ug.oy = g->origin[1];
@@ -471,7 +570,7 @@ int hs_read_strands(hs_ctx* c, uint32_t id, float* aos, hs_draw_indirect* ind)
         ug.omu = g->pend_omu;
         pug = &ug;
     }
-    hipError_t e = hs::launch_unpack(dev, g->S, g->N, g->Sp, g->pos, g->vel, g->corr, pug, c->stream);
+    hipError_t e = hs::launch_unpack(dev, g->perm, g->S, g->N, g->L.Sp, g->pos, g->vel, g->corr, pug, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(aos, dev, bytes, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(dev);
@@ -490,11 +589,11 @@ int hs_write_strands(hs_ctx* c, uint32_t id, const float* aos)
     float* dev = nullptr;
     HS_TRY(c, hipMalloc((void**)&dev, bytes));
     hipError_t e = hipMemcpyAsync(dev, aos, bytes, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = hs::launch_pack(dev, g->S, g->N, g->Sp, g->pos, g->vel, g->corr, c->stream);
+    if (e == hipSuccess) e = hs::launch_pack(dev, g->perm, g->S, g->N, g->L.Sp, g->pos, g->vel, g->corr, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(dev);
     if (e != hipSuccess) return hip_fail(c, e, "hs_write_strands");
-    g->pending = -1;
+    g->pending = false;
     return HS_OK;
 }
 
@@ -507,11 +606,11 @@ int hs_read_grid(hs_ctx* c, uint32_t id, hs_grid_cell* cells)
     if (g->G == 0) return fail(c, HS_ERR_STATE, "hs_read_grid: groom has no grid");
     HS_TRY(c, hipSetDevice(c->device));
     const size_t bytes = grid_cells(g->G) * 16;
-    if (g->pending < 0) {
+    if (!g->pending) {
         std::memset(cells, 0, bytes);
         return HS_OK;
     }
-    HS_TRY(c, hipMemcpyAsync(cells, g->grid[g->pending], bytes, hipMemcpyDeviceToHost, c->stream));
+    HS_TRY(c, hipMemcpyAsync(cells, g->gw.grid_i, bytes, hipMemcpyDeviceToHost, c->stream));
     HS_TRY(c, hipStreamSynchronize(c->stream));
     return HS_OK;
 }
@@ -523,7 +622,8 @@ int hs_profile_enable(hs_ctx* c, int on)
     if (rc != HS_OK) return rc;
     c->profile = on != 0;
     c->prof_launches = 0;
-    c->prof_ms = 0.0;
+    c->prof_step_ms = 0.0;
+    c->prof_reduce_ms = 0.0;
     return HS_OK;
 }
 
@@ -533,7 +633,17 @@ int hs_profile_read(hs_ctx* c, uint64_t* launches, double* total_ms)
     int rc = drain_profile(c);
     if (rc != HS_OK) return rc;
     *launches = c->prof_launches;
-    *total_ms = c->prof_ms;
+    *total_ms = c->prof_step_ms + c->prof_reduce_ms;
+    return HS_OK;
+}
+
+int hs_profile_breakdown(hs_ctx* c, double* step_kernel_ms, double* reduce_kernel_ms)
+{
+    if (!c || !step_kernel_ms || !reduce_kernel_ms) return HS_ERR_INVALID;
+    int rc = drain_profile(c);
+    if (rc != HS_OK) return rc;
+    *step_kernel_ms = c->prof_step_ms;
+    *reduce_kernel_ms = c->prof_reduce_ms;
     return HS_OK;
 }
 

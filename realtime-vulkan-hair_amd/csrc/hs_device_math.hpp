@@ -43,13 +43,15 @@ __device__ __forceinline__ f3 mat_vec(const float* m, f3 v, float w)
     return f3{o[0], o[1], o[2]};
 }
 
-// int(f) of the contract: truncation, saturating, NaN -> 0.
+// int(f) of the contract: truncation toward zero, saturating to
+// [INT32_MIN, INT32_MAX], NaN -> 0.  That is exactly what V_CVT_I32_F32 does
+// (out-of-range inputs clamp, NaN converts to 0), so the instruction is used
+// directly; a C cast would be undefined for the out-of-range cases.
 __device__ __forceinline__ int32_t sat_i32(float f)
 {
-    if (!(f == f)) return 0;
-    if (f >= 2147483648.0f) return INT32_MAX;
-    if (f < -2147483648.0f) return INT32_MIN;
-    return (int32_t)f;
+    int32_t r;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(f));
+    return r;
 }
 
 // Cells one grid coordinate touches (compute.comp:224-229):

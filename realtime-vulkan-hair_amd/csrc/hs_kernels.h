@@ -8,8 +8,24 @@
 
 namespace hs {
 
-constexpr uint32_t kStepBlock = 256;
+constexpr uint32_t kW = 128;          // strands per step workgroup (2 wavefronts)
+constexpr uint32_t kL = 16;           // points per segment of the root-to-tip sweep
+constexpr uint32_t kWinCells = 2048;  // LDS voxel window of one workgroup (16 B per cell)
+constexpr uint32_t kBrick = 8;        // reduce kernel: 8x8x8 voxels per workgroup
 constexpr uint32_t kMaxColliders = 8;
+constexpr uint32_t kDescInts = 8;     // partial-box descriptor: lo xyz, ext xyz, pool offset, pad
+
+// Device-side grid workspace of one groom (all zeroed at creation).
+struct GridWork {
+    int4* grid_i;         // G^3 GridCell ints: the reference-visible grid of the last step
+    float4* grid_f;       // G^3 (float(vx), float(vy), float(vz), density > 0 ? 1/float(density) : 0)
+    int4* pool;           // partial boxes written by the step kernel, [2][pool_cap]
+    int* desc;            // partial-box descriptors, [2][desc_cap][kDescInts]
+    uint32_t* counters;   // [2][4]: cells used, boxes used, overflow flag, pad
+    int4* overflow;       // G^3 int4 atomic fallback when the pool is full (normally zero)
+    int* gbox;            // [nwg][nseg][6] voxel box each workgroup-segment touched last step
+    uint32_t pool_cap, desc_cap;
+};
 
 // Everything one step launch needs, passed by value as the kernel argument.
 struct StepArgs {
@@ -18,20 +34,19 @@ struct StepArgs {
     float* corr;          // [3][N][Sp] or null (correction vectors not kept)
     const float* seg;     // [Sp] segment rest length
     const float* noise;   // [Sp] per-strand wind noise or null
-    uint32_t S, N, Sp;
+    uint32_t S, N, Sp, nwg, nseg;
     // voxel grid (compute.comp:9-11, 204-206)
-    const int4* gprev;    // grid of the previous step (its friction is applied here)
-    int4* gnext;          // grid this step splats into (zeroed)
-    int4* gclear;         // grid to zero for the next step, or null
-    uint32_t G, clear_cells;
+    GridWork gw;
+    uint32_t G, parity;   // parity selects pool/counter half written by this launch
     float ox, oy, oz, inv_h;
     float mu_prev, omu_prev;  // friction of the previous step and 1 - friction
     // step constants
-    float dt, inv_dt, dt2, D, k, vmax, sphere_r;
+    float dt, inv_dt, dt2, D, k, vmax, vmax2, sphere_r;  // vmax2: largest x with sqrt(x) <= vmax
     float fx, fy, fz;     // gravity + uniform wind
     float ndx, ndy, ndz;  // per-strand noise direction
-    uint32_t ncol;
-    Collider col[kMaxColliders];
+    float sphere_c[3];    // collider 0 centre (Collider.transform[3].xyz)
+    uint32_t ncol;        // colliders in use (0: none, 1: the sphere, >1: + ellipsoids)
+    const Collider* cols; // device copy of all ncol colliders (read for ellipsoids 1..ncol-1)
 };
 
 struct UnpackGrid {
@@ -41,9 +56,11 @@ struct UnpackGrid {
 };
 
 hipError_t launch_step(const StepArgs& a, bool gather, bool splat, hipStream_t st);
-hipError_t launch_pack(const float* aos_dev, uint32_t S, uint32_t N, uint32_t Sp, float* pos,
-                       float* vel, float* corr, hipStream_t st);
-hipError_t launch_unpack(float* aos_dev, uint32_t S, uint32_t N, uint32_t Sp, const float* pos,
-                         const float* vel, const float* corr, const UnpackGrid* g, hipStream_t st);
+hipError_t launch_reduce(const GridWork& gw, uint32_t G, uint32_t parity, hipStream_t st);
+hipError_t launch_pack(const float* aos_dev, const uint32_t* perm, uint32_t S, uint32_t N, uint32_t Sp,
+                       float* pos, float* vel, float* corr, hipStream_t st);
+hipError_t launch_unpack(float* aos_dev, const uint32_t* perm, uint32_t S, uint32_t N, uint32_t Sp,
+                         const float* pos, const float* vel, const float* corr, const UnpackGrid* g,
+                         hipStream_t st);
 
 }  // namespace hs

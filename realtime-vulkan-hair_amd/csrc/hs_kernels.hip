@@ -1,22 +1,32 @@
 // hs_kernels.hip - HIP kernels of the strand-physics pass for gfx950.
 //
 // State layout in HBM (strand-major SoA, one plane per component):
-//   pos[c][i][s], vel[c][i][s], c in {x,y,z}, i = point 0..N-1, s = strand
-//   0..Sp-1 (Sp = S rounded up to 64).  At a fixed point index the 64 lanes of
-//   a wavefront read 64 consecutive floats of each plane: one 256 B line.
-// One lane owns one strand and sweeps its points root to tip, keeping only
-// the parent position and the previous point's pending velocity in registers
-// (the DFTL correction of point i-1 needs point i's correction vector).
+//   pos[c][i][p], vel[c][i][p], c in {x,y,z}, i = point 0..N-1, p = physical
+//   strand slot 0..Sp-1.  Strands are stored in Morton order of their roots
+//   (perm[p] = caller's strand index), so the kW strands of one workgroup are
+//   spatial neighbours.  At a fixed point index a wavefront reads 64
+//   consecutive floats of each plane: one 256 B line per plane.
 //
-// The reference's step (compute.comp:124-303) has a global dependency inside
-// it: every strand's splat must land before any strand gathers.  The kernels
-// here are step-skewed: launch k applies step k-1's grid friction to each
-// point as it is loaded (gather from grid[k-1]) and then runs step k's
-// predict / FTL / DFTL / splat (into grid[k]).  Positions written by launch k
-// are the positions after step k; stored velocities are the post-DFTL
-// velocities of step k, whose friction is applied by launch k+1 or by the
-// read-back kernel.  Per point the operations and their order are exactly
-// those of the unskewed step, so results are bit-identical.
+// One step = two kernels.
+//  k_step   (one workgroup per kW strands) walks each strand root to tip in
+//           segments of kL points held in registers.  Per segment:
+//             P: per point, friction with the previous step's voxel window
+//                staged in LDS (compute.comp:257-298), force and predicted
+//                position (compute.comp:150-187), all points independent;
+//             F: follow-the-leader, damped velocity, DFTL (compute.comp:
+//                190-215), the root-to-tip chain;
+//             S: trilinear fixed-point splat (compute.comp:217-252) of the
+//                points whose velocity is final into an LDS window, dumped as
+//                a dense partial box to a pool.
+//  k_reduce (one workgroup per 8^3 voxel brick) sums every partial box that
+//           overlaps its brick and writes the step's grid: the GridCell ints
+//           and the float cells the next step's G phase reads.  It writes every
+//           voxel, which also clears the grid (the vkCmdFillBuffer of
+//           Renderer.cpp:2063).
+// The step is skewed: the friction of step k is applied by k_step of step
+// k+1 (or by the read-back kernel).  Per point the operations and their order
+// are those of the unskewed shader, so results are bit-identical; integer
+// voxel sums are order-independent (wrapping adds), so the grid is exact too.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -25,30 +35,132 @@
 
 namespace hs {
 
-// ---------------------------------------------------------------------------
-// Grid helpers
-// ---------------------------------------------------------------------------
+static_assert((uint64_t)kW * (kL + 1) * 1000000ull < (1ull << 32),
+              "packed (vx<<32 | density) LDS sums need the density of one window below 2^32");
+static_assert(kW % 64 == 0, "workgroup is whole wavefronts");
+static_assert(kL <= 31, "per-point flags live in one 32-bit mask");
 
-// Grid velocity at grid coordinate q (compute.comp:264-293): sum over the
-// touched cells with density > 0 of (w * (1/float(density))) * vec3(vel),
-// cells visited x-major, then y, then z as the shader's a/b/c loops.
-__device__ __forceinline__ f3 grid_gather(const int4* __restrict__ g, f3 q, int G)
+// ---------------------------------------------------------------------------
+// Voxel ranges
+// ---------------------------------------------------------------------------
+struct Rng {
+    int x0, x1, y0, y1, z0, z1;  // inclusive; empty when any lo > hi
+};
+
+__device__ __forceinline__ f3 grid_coord(f3 p, f3 origin, float inv_h) { return muls(sub(p, origin), inv_h); }
+
+__device__ __forceinline__ Rng point_range(f3 q, int G)
 {
-    int x0, x1, y0, y1, z0, z1;
-    cell_range(q.x, G, x0, x1);
-    cell_range(q.y, G, y0, y1);
-    cell_range(q.z, G, z0, z1);
+    Rng r;
+    cell_range(q.x, G, r.x0, r.x1);
+    cell_range(q.y, G, r.y0, r.y1);
+    cell_range(q.z, G, r.z0, r.z1);
+    return r;
+}
+
+__device__ __forceinline__ bool rng_empty(const Rng& r) { return r.x0 > r.x1 || r.y0 > r.y1 || r.z0 > r.z1; }
+
+__device__ __forceinline__ bool rng_full(const Rng& r)
+{
+    return r.x1 == r.x0 + 1 && r.y1 == r.y0 + 1 && r.z1 == r.z0 + 1;
+}
+
+// Friction blend (compute.comp:296-297): (1 - mu) * v + mu * v_grid.
+__device__ __forceinline__ f3 friction(f3 v, f3 gv, float mu, float omu) { return add(smul(omu, v), smul(mu, gv)); }
+
+// A window of voxels staged in LDS: cell (a,b,c) at ((c-z0)*ey + (b-y0))*ex + (a-x0).
+struct Win {
+    int x0, y0, z0, ex, ey, ez;
+};
+
+__device__ __forceinline__ bool in_win(const Rng& r, const Win& w)
+{
+    return r.x0 >= w.x0 && r.x1 < w.x0 + w.ex && r.y0 >= w.y0 && r.y1 < w.y0 + w.ey && r.z0 >= w.z0 &&
+           r.z1 < w.z0 + w.ez;
+}
+
+__device__ __forceinline__ int win_index(const Win& w, int a, int b, int c)
+{
+    return ((c - w.z0) * w.ey + (b - w.y0)) * w.ex + (a - w.x0);
+}
+
+
+// ---------------------------------------------------------------------------
+// Grid gathers.  Cells are visited x-major, then y, then z as the shader's
+// a/b/c loops; a float cell holds (vx, vy, vz, 1/density) with 1/density = 0
+// for density <= 0, so a skipped cell adds +-0, which never changes the sum
+// (the sum starts at +0 and round-to-nearest never produces -0 from x + (-x)).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void gather_cell(f3& gv, float4 c, float w)
+{
+    const float t = w * c.w;
+    gv = add(gv, smul(t, mk(c.x, c.y, c.z)));
+}
+
+__device__ __forceinline__ float wgt(float q, int a) { return clamp01(1.0f - fabsf(q - (float)a)); }
+
+// Full 2x2x2 block inside an LDS window of float cells.
+__device__ __forceinline__ f3 gather_block(const float4* win, const Win& wd, const Rng& r, f3 q)
+{
+    const float wx0 = wgt(q.x, r.x0), wx1 = wgt(q.x, r.x1);
+    const float wy0 = wgt(q.y, r.y0), wy1 = wgt(q.y, r.y1);
+    const float wz0 = wgt(q.z, r.z0), wz1 = wgt(q.z, r.z1);
+    const int sy = wd.ex, sz = wd.ex * wd.ey;
+    const float4* b = win + win_index(wd, r.x0, r.y0, r.z0);
+    const float4 c000 = b[0], c001 = b[sz], c010 = b[sy], c011 = b[sy + sz];
+    const float4 c100 = b[1], c101 = b[1 + sz], c110 = b[1 + sy], c111 = b[1 + sy + sz];
     f3 gv = mk(0.0f, 0.0f, 0.0f);
-    for (int a = x0; a <= x1; ++a) {
-        const float wx = clamp01(1.0f - fabsf(q.x - (float)a));
-        for (int b = y0; b <= y1; ++b) {
-            const float wy = clamp01(1.0f - fabsf(q.y - (float)b));
-            const float wxy = wx * wy;
-            for (int c = z0; c <= z1; ++c) {
+    float wxy = wx0 * wy0;
+    gather_cell(gv, c000, wxy * wz0);
+    gather_cell(gv, c001, wxy * wz1);
+    wxy = wx0 * wy1;
+    gather_cell(gv, c010, wxy * wz0);
+    gather_cell(gv, c011, wxy * wz1);
+    wxy = wx1 * wy0;
+    gather_cell(gv, c100, wxy * wz0);
+    gather_cell(gv, c101, wxy * wz1);
+    wxy = wx1 * wy1;
+    gather_cell(gv, c110, wxy * wz0);
+    gather_cell(gv, c111, wxy * wz1);
+    return gv;
+}
+
+// Any non-empty block (clipped at the grid boundary, or outside the window):
+// from the LDS window when `lds`, else from the global float grid.
+__device__ __forceinline__ f3 gather_any(const float4* win, const Win& wd, bool lds, const float4* __restrict__ g,
+                                         const Rng& r, f3 q, int G)
+{
+    f3 gv = mk(0.0f, 0.0f, 0.0f);
+#pragma unroll 1
+    for (int a = r.x0; a <= r.x1; ++a) {
+        const float wxa = wgt(q.x, a);
+#pragma unroll 1
+        for (int b = r.y0; b <= r.y1; ++b) {
+            const float wxy = wxa * wgt(q.y, b);
+#pragma unroll 1
+            for (int c = r.z0; c <= r.z1; ++c) {
+                const float w = wxy * wgt(q.z, c);
+                const float4 cell =
+                    lds ? win[win_index(wd, a, b, c)] : g[(size_t)a + (size_t)b * G + (size_t)c * G * G];
+                gather_cell(gv, cell, w);
+            }
+        }
+    }
+    return gv;
+}
+
+// From the integer grid, literally as compute.comp:274-293 (read-back path).
+__device__ __forceinline__ f3 gather_global_i(const int4* __restrict__ g, const Rng& r, f3 q, int G)
+{
+    f3 gv = mk(0.0f, 0.0f, 0.0f);
+    for (int a = r.x0; a <= r.x1; ++a) {
+        const float wxa = wgt(q.x, a);
+        for (int b = r.y0; b <= r.y1; ++b) {
+            const float wxy = wxa * wgt(q.y, b);
+            for (int c = r.z0; c <= r.z1; ++c) {
                 const int4 cell = g[(size_t)a + (size_t)b * G + (size_t)c * G * G];
                 if (cell.w > 0) {
-                    const float wz = clamp01(1.0f - fabsf(q.z - (float)c));
-                    const float w = wxy * wz;
+                    const float w = wxy * wgt(q.z, c);
                     const float t = w * (1.0f / (float)cell.w);
                     gv = add(gv, smul(t, mk((float)cell.x, (float)cell.y, (float)cell.z)));
                 }
@@ -58,65 +170,115 @@ __device__ __forceinline__ f3 grid_gather(const int4* __restrict__ g, f3 q, int 
     return gv;
 }
 
-// Trilinear fixed-point splat at grid coordinate q (compute.comp:224-252).
-__device__ __forceinline__ void grid_splat(int4* __restrict__ g, f3 q, f3 v, int G)
+// ---------------------------------------------------------------------------
+// Splat.  Contributions int(SCALE*(w*v)) and int(SCALE*w) (compute.comp:241-248)
+// accumulate in an LDS window of kSlots copies per voxel (lane & (kSlots-1)
+// picks the copy, which spreads neighbouring lanes hitting one voxel over
+// different addresses).  A copy is 16 B:
+//   u64 (vx << 32 | density), u32 vy, u32 vz.
+// Density addends are in [0, 1e6] and one window holds at most kW*(kL+1)
+// points, so the low word never carries into vx: both sums are exact mod 2^32.
+// ---------------------------------------------------------------------------
+constexpr int kSlots = 2;
+constexpr int kSplatCells = (int)kWinCells / kSlots;  // voxels per splat window
+
+__device__ __forceinline__ void splat_cell(unsigned char* cellp, float w, f3 v)
 {
     const float SCALE = 1000000.0f;
-    int x0, x1, y0, y1, z0, z1;
-    cell_range(q.x, G, x0, x1);
-    cell_range(q.y, G, y0, y1);
-    cell_range(q.z, G, z0, z1);
-    for (int a = x0; a <= x1; ++a) {
-        const float wx = clamp01(1.0f - fabsf(q.x - (float)a));
-        for (int b = y0; b <= y1; ++b) {
-            const float wy = clamp01(1.0f - fabsf(q.y - (float)b));
-            const float wxy = wx * wy;
-            for (int c = z0; c <= z1; ++c) {
-                const float wz = clamp01(1.0f - fabsf(q.z - (float)c));
-                const float w = wxy * wz;
-                const f3 wv = smul(w, v);
-                int* cell = reinterpret_cast<int*>(g + ((size_t)a + (size_t)b * G + (size_t)c * G * G));
-                __hip_atomic_fetch_add(cell + 0, sat_i32(SCALE * wv.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_fetch_add(cell + 1, sat_i32(SCALE * wv.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_fetch_add(cell + 2, sat_i32(SCALE * wv.z), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_fetch_add(cell + 3, sat_i32(SCALE * w), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const f3 wv = smul(w, v);
+    const uint32_t ix = (uint32_t)sat_i32(SCALE * wv.x);
+    const uint32_t iy = (uint32_t)sat_i32(SCALE * wv.y);
+    const uint32_t iz = (uint32_t)sat_i32(SCALE * wv.z);
+    const uint32_t id = (uint32_t)sat_i32(SCALE * w);
+    atomicAdd(reinterpret_cast<unsigned long long*>(cellp), ((unsigned long long)ix << 32) | id);
+    atomicAdd(reinterpret_cast<uint32_t*>(cellp + 8), iy);
+    atomicAdd(reinterpret_cast<uint32_t*>(cellp + 12), iz);
+}
+
+// byte address of voxel (a,b,c)'s copy for this lane
+__device__ __forceinline__ unsigned char* splat_addr(int4* win, const Win& wd, int a, int b, int c, int slot)
+{
+    return reinterpret_cast<unsigned char*>(win) + 16 * (win_index(wd, a, b, c) * kSlots + slot);
+}
+
+// Full 2x2x2 block inside the LDS window.
+__device__ __forceinline__ void splat_block(int4* win, const Win& wd, const Rng& r, f3 q, f3 v, int slot)
+{
+    const float wx0 = wgt(q.x, r.x0), wx1 = wgt(q.x, r.x1);
+    const float wy0 = wgt(q.y, r.y0), wy1 = wgt(q.y, r.y1);
+    const float wz0 = wgt(q.z, r.z0), wz1 = wgt(q.z, r.z1);
+    const int sx = 16 * kSlots, sy = sx * wd.ex, sz = sy * wd.ey;
+    unsigned char* b = splat_addr(win, wd, r.x0, r.y0, r.z0, slot);
+    float wxy = wx0 * wy0;
+    splat_cell(b, wxy * wz0, v);
+    splat_cell(b + sz, wxy * wz1, v);
+    wxy = wx0 * wy1;
+    splat_cell(b + sy, wxy * wz0, v);
+    splat_cell(b + sy + sz, wxy * wz1, v);
+    wxy = wx1 * wy0;
+    splat_cell(b + sx, wxy * wz0, v);
+    splat_cell(b + sx + sz, wxy * wz1, v);
+    wxy = wx1 * wy1;
+    splat_cell(b + sx + sy, wxy * wz0, v);
+    splat_cell(b + sx + sy + sz, wxy * wz1, v);
+}
+
+// Any non-empty block: into the LDS window when `lds`, else device atomics
+// into the global overflow grid (when no window fits).
+__device__ __forceinline__ void splat_any(int4* win, const Win& wd, bool lds, int slot, int4* __restrict__ g,
+                                          const Rng& r, f3 q, f3 v, int G)
+{
+    const float SCALE = 1000000.0f;
+#pragma unroll 1
+    for (int a = r.x0; a <= r.x1; ++a) {
+        const float wxa = wgt(q.x, a);
+#pragma unroll 1
+        for (int b = r.y0; b <= r.y1; ++b) {
+            const float wxy = wxa * wgt(q.y, b);
+#pragma unroll 1
+            for (int c = r.z0; c <= r.z1; ++c) {
+                const float w = wxy * wgt(q.z, c);
+                if (lds) {
+                    splat_cell(splat_addr(win, wd, a, b, c, slot), w, v);
+                } else {
+                    const f3 wv = smul(w, v);
+                    int* cell = reinterpret_cast<int*>(g + ((size_t)a + (size_t)b * G + (size_t)c * G * G));
+                    atomicAdd(cell + 0, sat_i32(SCALE * wv.x));
+                    atomicAdd(cell + 1, sat_i32(SCALE * wv.y));
+                    atomicAdd(cell + 2, sat_i32(SCALE * wv.z));
+                    atomicAdd(cell + 3, sat_i32(SCALE * w));
+                }
             }
         }
     }
 }
 
-__device__ __forceinline__ f3 grid_coord(f3 p, f3 origin, float inv_h)
-{
-    return muls(sub(p, origin), inv_h);
-}
-
-// Friction blend (compute.comp:296-297): (1 - mu) * v + mu * v_grid.
-__device__ __forceinline__ f3 friction(f3 v, f3 gv, float mu, float omu)
-{
-    return add(smul(omu, v), smul(mu, gv));
-}
-
 // ---------------------------------------------------------------------------
-// Collider penalty (compute.comp:155-184)
+// Force at a start-of-step position: gravity (+ wind, + strand noise) then the
+// collider penalty loop (compute.comp:150-184).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ f3 collide(f3 x, f3 force, const StepArgs& a)
+template <bool ELL>
+__device__ __forceinline__ f3 force_at(f3 x, f3 fs, const StepArgs& a)
 {
     f3 acc = mk(0.0f, 0.0f, 0.0f);
     int hits = 0;
-    for (uint32_t j = 0; j < a.ncol; ++j) {
-        const float* T = a.col[j].m;
-        if (j == 0) {
-            // sphere: compute.comp:160-169 (centre = transform[3].xyz)
-            const f3 dv = sub(x, mk(T[12], T[13], T[14]));
-            const float dist = length(dv);
-            if (dist < a.sphere_r) {
-                const float d = a.sphere_r - dist;
-                const f3 n = muls(dv, 1.0f / dist);
-                acc = add(acc, smul(a.k * d, n));
-                hits += 1;
-            }
-        } else {
-            // ellipsoid: compute.comp:64-79, 170-179
+    if (a.ncol > 0) {
+        // sphere: compute.comp:160-169 (centre = transform[3].xyz)
+        const f3 dv = sub(x, mk(a.sphere_c[0], a.sphere_c[1], a.sphere_c[2]));
+        const float dist = length(dv);
+        if (dist < a.sphere_r) {
+            const float d = a.sphere_r - dist;
+            const f3 n = muls(dv, 1.0f / dist);
+            acc = add(acc, smul(a.k * d, n));
+            hits = 1;
+        }
+    }
+    if (ELL) {
+        // ellipsoids: EllipsoidCollision / GetPointOnEllipsoid /
+        // GetEllipsoidNormal (compute.comp:64-79, 170-179)
+#pragma unroll 1
+        for (uint32_t j = 1; j < a.ncol; ++j) {
+            const float* T = a.cols[j].m;
             const f3 tp = mat_vec(T + 16, x, 1.0f);
             if (length(tp) <= 1.0f) {
                 const f3 surf = mat_vec(T, normalize(tp), 1.0f);
@@ -127,6 +289,7 @@ __device__ __forceinline__ f3 collide(f3 x, f3 force, const StepArgs& a)
             }
         }
     }
+    f3 force = fs;
     if (hits > 0) {
         const float m = (float)hits;
         force = add(force, mk(acc.x / m, acc.y / m, acc.z / m));
@@ -135,15 +298,159 @@ __device__ __forceinline__ f3 collide(f3 x, f3 force, const StepArgs& a)
 }
 
 // ---------------------------------------------------------------------------
+// Workgroup reductions and window planning
+// ---------------------------------------------------------------------------
+constexpr int kWaves = kW / 64;
+
+__device__ __forceinline__ int wave_min(int v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = min(v, __shfl_xor(v, m, 64));
+    return v;
+}
+
+// Box of voxels (inclusive bounds).  Empty: lo = INT_MAX, hi = INT_MIN.
+struct Box {
+    int lo[3], hi[3];
+};
+
+__device__ __forceinline__ void box_add(Box& b, const Rng& r)
+{
+    if (rng_empty(r)) return;
+    b.lo[0] = min(b.lo[0], r.x0);
+    b.hi[0] = max(b.hi[0], r.x1);
+    b.lo[1] = min(b.lo[1], r.y0);
+    b.hi[1] = max(b.hi[1], r.y1);
+    b.lo[2] = min(b.lo[2], r.z0);
+    b.hi[2] = max(b.hi[2], r.z1);
+}
+
+__device__ __forceinline__ Box box_empty()
+{
+    Box b;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        b.lo[k] = INT32_MAX;
+        b.hi[k] = INT32_MIN;
+    }
+    return b;
+}
+
+__device__ __forceinline__ bool box_is_empty(const Box& b)
+{
+    return b.lo[0] > b.hi[0] || b.lo[1] > b.hi[1] || b.lo[2] > b.hi[2];
+}
+
+// Reduce a box over the workgroup; every thread gets the result.
+__device__ __forceinline__ Box block_box(Box b, int* scratch)
+{
+    int v[6];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        v[k] = b.lo[k];
+        v[k + 3] = b.hi[k] == INT32_MIN ? INT32_MAX : -b.hi[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] = wave_min(v[k]);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) scratch[wave * 6 + k] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        int m = scratch[k];
+        for (int w = 1; w < kWaves; ++w) m = min(m, scratch[w * 6 + k]);
+        v[k] = m;
+    }
+    __syncthreads();
+    Box r;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        r.lo[k] = v[k];
+        r.hi[k] = v[k + 3] == INT32_MAX ? INT32_MIN : -v[k + 3];
+    }
+    return r;
+}
+
+// Chunking of a box into LDS windows of at most `cap` voxels: full x extent,
+// chunks in y and z overlapping by one voxel so that every 2x2x2 block of the
+// box lies whole inside at least one chunk.
+struct Plan {
+    int ex, ey, ez;   // box extents
+    int cy, cz;       // chunk extents in y and z (cells)
+    int ny, nz;       // chunk counts
+    bool ok;          // false: no chunking fits (or empty box)
+};
+
+__device__ __forceinline__ Plan plan_box(const Box& b, int cap)
+{
+    Plan p;
+    if (box_is_empty(b)) {
+        p.ex = p.ey = p.ez = 0;
+        p.cy = p.cz = 2;
+        p.ny = p.nz = 0;
+        p.ok = false;
+        return p;
+    }
+    p.ex = b.hi[0] - b.lo[0] + 1;
+    p.ey = b.hi[1] - b.lo[1] + 1;
+    p.ez = b.hi[2] - b.lo[2] + 1;
+    p.ok = true;
+    if (p.ex * p.ey * p.ez <= cap) {
+        p.cy = p.ey;
+        p.cz = p.ez;
+    } else if (p.ex * p.ey * 2 <= cap) {
+        p.cy = p.ey;
+        p.cz = cap / (p.ex * p.ey);
+    } else if (p.ex * 4 <= cap) {
+        p.cz = 2;
+        p.cy = cap / (p.ex * 2);
+    } else {
+        p.ok = false;
+        p.cy = p.cz = 2;
+    }
+    p.ny = p.cy >= p.ey ? 1 : 1 + (p.ey - 2) / (p.cy - 1);
+    p.nz = p.cz >= p.ez ? 1 : 1 + (p.ez - 2) / (p.cz - 1);
+    return p;
+}
+
+__device__ __forceinline__ Win chunk_of(const Box& b, const Plan& p, int iy, int iz)
+{
+    Win w;
+    w.x0 = b.lo[0];
+    w.ex = p.ex;
+    w.y0 = b.lo[1] + iy * (p.cy - 1);
+    w.z0 = b.lo[2] + iz * (p.cz - 1);
+    w.ey = min(p.cy, b.hi[1] - w.y0 + 1);
+    w.ez = min(p.cz, b.hi[2] - w.z0 + 1);
+    return w;
+}
+
+// ---------------------------------------------------------------------------
 // The step kernel
 // ---------------------------------------------------------------------------
-template <bool GATHER, bool SPLAT>
-__global__ __launch_bounds__(kStepBlock) void k_step(StepArgs a)
+template <bool GATHER, bool SPLAT, bool ELL>
+__global__ __launch_bounds__(kW) void k_step(StepArgs a)
 {
-    const uint32_t s = blockIdx.x * kStepBlock + threadIdx.x;
+    __shared__ __align__(16) int4 win[kWinCells];
+    __shared__ int red[kWaves * 6];
+    __shared__ uint32_t bcast[2];
+
+    // XCD-aware order: dispatch places consecutive blockIdx on different XCDs
+    // (round robin over 8); give each XCD a contiguous run of Morton-ordered
+    // strand blocks so neighbouring windows share that XCD's L2.
+    const uint32_t per = (a.nwg + 7) / 8;
+    const uint32_t wg = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (wg >= a.nwg) return;
+
+    const uint32_t s = wg * kW + threadIdx.x;
+    const bool active = s < a.S;
+    const int slot = threadIdx.x & (kSlots - 1);
     const uint32_t Sp = a.Sp;
     const uint32_t N = a.N;
-    const size_t plane = (size_t)N * Sp;
+    const uint32_t plane = N * Sp;  // < 2^30 (checked on the host)
     float* __restrict__ px = a.pos;
     float* __restrict__ py = a.pos + plane;
     float* __restrict__ pz = a.pos + 2 * plane;
@@ -153,91 +460,420 @@ __global__ __launch_bounds__(kStepBlock) void k_step(StepArgs a)
     const f3 origin = mk(a.ox, a.oy, a.oz);
     const int G = (int)a.G;
 
-    if (s < a.S) {
-        f3 par = mk(px[s], py[s], pz[s]);
-        const float r = a.seg[s];
-        f3 fs = mk(a.fx, a.fy, a.fz);
+    f3 par = mk(0.0f, 0.0f, 0.0f);
+    float r = 0.0f;
+    f3 fs = mk(a.fx, a.fy, a.fz);
+    if (active) {
+        par = mk(px[s], py[s], pz[s]);
+        r = a.seg[s];
         if (a.noise) fs = add(fs, smul(a.noise[s], mk(a.ndx, a.ndy, a.ndz)));
-
-        f3 vpend = mk(0.0f, 0.0f, 0.0f);   // velocity of point i-1 awaiting DFTL
-        f3 ppend = mk(0.0f, 0.0f, 0.0f);   // position of point i-1
-        for (uint32_t i = 1; i < N; ++i) {
-            const size_t o = (size_t)i * Sp + s;
-            const f3 x = mk(px[o], py[o], pz[o]);
-            f3 u = mk(vx[o], vy[o], vz[o]);
-            if (GATHER) {
-                // previous step's phase 3 (compute.comp:257-298) for this point
-                const f3 gv = grid_gather(a.gprev, grid_coord(x, origin, a.inv_h), G);
-                u = friction(u, gv, a.mu_prev, a.omu_prev);
-            }
-            const f3 f = collide(x, fs, a);
-            // predict (compute.comp:187)
-            const f3 pred = add(add(x, smul(a.dt, u)), smul(a.dt2, f));
-            // follow the leader (compute.comp:191-192)
-            const f3 pn = add(par, smul(r, normalize(sub(pred, par))));
-            // damped velocity, clamp, correction (compute.comp:195-201)
-            f3 vel = smul(a.D, muls(sub(pn, x), a.inv_dt));
-            if (length(vel) > a.vmax) vel = muls(normalize(vel), a.vmax);
-            const f3 corr = smul(a.D, sub(pn, pred));
-            px[o] = pn.x;
-            py[o] = pn.y;
-            pz[o] = pn.z;
-            if (a.corr) {
-                a.corr[o] = corr.x;
-                a.corr[plane + o] = corr.y;
-                a.corr[2 * plane + o] = corr.z;
-            }
-            if (i >= 2) {
-                // DFTL for point i-1 (compute.comp:213-215), then its splat
-                vpend = sub(vpend, muls(corr, a.inv_dt));
-                const size_t op = o - Sp;
-                vx[op] = vpend.x;
-                vy[op] = vpend.y;
-                vz[op] = vpend.z;
-                if (SPLAT) grid_splat(a.gnext, grid_coord(ppend, origin, a.inv_h), vpend, G);
-            }
-            vpend = vel;
-            ppend = pn;
-            par = pn;
-        }
-        // tip: no correction
-        const size_t ot = (size_t)(N - 1) * Sp + s;
-        vx[ot] = vpend.x;
-        vy[ot] = vpend.y;
-        vz[ot] = vpend.z;
-        if (SPLAT) grid_splat(a.gnext, grid_coord(ppend, origin, a.inv_h), vpend, G);
     }
+    // point f-1 of the previous segment: position final, velocity awaiting DFTL
+    f3 cp = mk(0.0f, 0.0f, 0.0f), cv = mk(0.0f, 0.0f, 0.0f);
 
-    // Clear the grid buffer the next launch will splat into.
-    if (a.gclear) {
-        const uint32_t nthreads = gridDim.x * kStepBlock;
-        const uint32_t t = blockIdx.x * kStepBlock + threadIdx.x;
-        for (uint32_t c = t; c < a.clear_cells; c += nthreads) a.gclear[c] = make_int4(0, 0, 0, 0);
+    for (uint32_t sg = 0; sg < a.nseg; ++sg) {
+        const uint32_t f = 1 + sg * kL;
+        const uint32_t cnt = min(kL, N - f);
+        const bool last_seg = f + cnt == N;
+        const uint32_t o0 = f * Sp + s;
+        f3 X[kL], U[kL];
+#pragma unroll
+        for (uint32_t t = 0; t < kL; ++t) {
+            X[t] = mk(0.0f, 0.0f, 0.0f);
+            U[t] = mk(0.0f, 0.0f, 0.0f);
+            if (active && t < cnt) {
+                const uint32_t o = o0 + t * Sp;
+                X[t] = mk(px[o], py[o], pz[o]);
+                U[t] = mk(vx[o], vy[o], vz[o]);
+            }
+        }
+
+        // ---------------- G: stage the previous step's voxel window ----------------
+        Win gwin{0, 0, 0, 0, 0, 0};
+        bool gwin_ok = false;
+        if (GATHER) {
+            const int* gb = a.gw.gbox + ((size_t)wg * a.nseg + sg) * 6;
+            Box box;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                box.lo[k] = gb[k];
+                box.hi[k] = gb[3 + k];
+            }
+            const Plan pl = plan_box(box, (int)kWinCells);
+            if (pl.ok && pl.ny == 1 && pl.nz == 1) {
+                gwin = chunk_of(box, pl, 0, 0);
+                gwin_ok = true;
+                const int cells = gwin.ex * gwin.ey * gwin.ez;
+                float4* fw = reinterpret_cast<float4*>(win);
+                // loads for up to 8 cells per thread in flight before the LDS stores
+                for (int c0 = 0; c0 < cells; c0 += 8 * kW) {
+                    float4 tmp[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        const int c = c0 + k * kW + threadIdx.x;
+                        if (c < cells) {
+                            const int ax = c % gwin.ex;
+                            const int rest = c / gwin.ex;
+                            tmp[k] = a.gw.grid_f[(size_t)(gwin.x0 + ax) + (size_t)(gwin.y0 + rest % gwin.ey) * G +
+                                                 (size_t)(gwin.z0 + rest / gwin.ey) * G * G];
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        const int c = c0 + k * kW + threadIdx.x;
+                        if (c < cells) fw[c] = tmp[k];
+                    }
+                }
+                __syncthreads();
+            }
+        }
+
+        // ------- P: friction, force and predicted position of every point -------
+        if (active) {
+#pragma unroll
+            for (uint32_t t = 0; t < kL; ++t) {
+                if (t >= cnt) continue;
+                const f3 x = X[t];
+                f3 u = U[t];
+                if (GATHER) {
+                    // previous step's phase 3 (compute.comp:257-298) for this point
+                    const f3 q = grid_coord(x, origin, a.inv_h);
+                    const Rng rg = point_range(q, G);
+                    f3 gv = mk(0.0f, 0.0f, 0.0f);
+                    const bool inw = gwin_ok && in_win(rg, gwin);
+                    if (inw && rng_full(rg))
+                        gv = gather_block(reinterpret_cast<const float4*>(win), gwin, rg, q);
+                    else if (!rng_empty(rg))
+                        gv = gather_any(reinterpret_cast<const float4*>(win), gwin, inw, a.gw.grid_f, rg, q, G);
+                    u = friction(u, gv, a.mu_prev, a.omu_prev);
+                }
+                // predict (compute.comp:187)
+                const f3 fc = force_at<ELL>(x, fs, a);
+                U[t] = add(add(x, smul(a.dt, u)), smul(a.dt2, fc));
+            }
+        }
+        if (GATHER && gwin_ok) __syncthreads();  // the window is reused below
+
+        // ---------------- F: FTL, velocity, DFTL (the root-to-tip chain) ----------------
+        Box nbox = box_empty();  // voxels of this segment's new positions (next step's gather)
+        if (active) {
+#pragma unroll
+            for (uint32_t t = 0; t < kL; ++t) {
+                if (t >= cnt) continue;
+                const uint32_t o = o0 + t * Sp;
+                const f3 x = X[t];
+                const f3 pred = U[t];
+                // follow the leader (compute.comp:191-192)
+                const f3 pn = add(par, smul(r, normalize(sub(pred, par))));
+                // damped velocity, clamp, correction (compute.comp:195-201);
+                // dot > vmax2 is exactly sqrt(dot) > vmax
+                f3 vel = smul(a.D, muls(sub(pn, x), a.inv_dt));
+                if (dot(vel, vel) > a.vmax2) vel = muls(normalize(vel), a.vmax);
+                const f3 corr = smul(a.D, sub(pn, pred));
+                px[o] = pn.x;
+                py[o] = pn.y;
+                pz[o] = pn.z;
+                if (a.corr) {
+                    a.corr[o] = corr.x;
+                    a.corr[plane + o] = corr.y;
+                    a.corr[2 * plane + o] = corr.z;
+                }
+                // DFTL for point f+t-1 (compute.comp:213-215)
+                const f3 dc = muls(corr, a.inv_dt);
+                if (t == 0) {
+                    if (f >= 2) {
+                        cv = sub(cv, dc);
+                        vx[o - Sp] = cv.x;
+                        vy[o - Sp] = cv.y;
+                        vz[o - Sp] = cv.z;
+                    }
+                } else {
+                    U[t - 1] = sub(U[t - 1], dc);
+                    vx[o - Sp] = U[t - 1].x;
+                    vy[o - Sp] = U[t - 1].y;
+                    vz[o - Sp] = U[t - 1].z;
+                }
+                X[t] = pn;
+                U[t] = vel;
+                par = pn;
+                if (SPLAT) box_add(nbox, point_range(grid_coord(pn, origin, a.inv_h), G));
+                if (t + 1 == cnt && last_seg) {
+                    // tip: no correction (compute.comp:213)
+                    vx[o] = vel.x;
+                    vy[o] = vel.y;
+                    vz[o] = vel.z;
+                }
+            }
+        }
+
+        // ---------------- S: splat of the finished points ----------------
+        if (SPLAT) {
+            // finished: point f-1 (carried), points f..f+cnt-2, and the tip
+            Box sbox = box_empty();
+            if (active) {
+                if (f >= 2) box_add(sbox, point_range(grid_coord(cp, origin, a.inv_h), G));
+#pragma unroll
+                for (uint32_t t = 0; t < kL; ++t)
+                    if (t + 1 < cnt || (t + 1 == cnt && last_seg))
+                        box_add(sbox, point_range(grid_coord(X[t], origin, a.inv_h), G));
+            }
+            const Box gbn = block_box(nbox, red);
+            if (threadIdx.x == 0) {
+                int* gb = a.gw.gbox + ((size_t)wg * a.nseg + sg) * 6;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    gb[k] = gbn.lo[k];
+                    gb[3 + k] = gbn.hi[k];
+                }
+            }
+            const Box box = block_box(sbox, red);
+            const Plan pl = plan_box(box, kSplatCells);
+            uint32_t* ctr = a.gw.counters + a.parity * 4;
+            if (!box_is_empty(box) && !pl.ok) {
+                // no window fits: device atomics into the overflow grid
+                const Win none{0, 0, 0, 0, 0, 0};
+                if (active) {
+                    if (f >= 2) {
+                        const f3 q = grid_coord(cp, origin, a.inv_h);
+                        const Rng rg = point_range(q, G);
+                        if (!rng_empty(rg)) splat_any(win, none, false, slot, a.gw.overflow, rg, q, cv, G);
+                    }
+#pragma unroll
+                    for (uint32_t t = 0; t < kL; ++t) {
+                        if (!(t + 1 < cnt || (t + 1 == cnt && last_seg))) continue;
+                        const f3 q = grid_coord(X[t], origin, a.inv_h);
+                        const Rng rg = point_range(q, G);
+                        if (!rng_empty(rg)) splat_any(win, none, false, slot, a.gw.overflow, rg, q, U[t], G);
+                    }
+                }
+                if (threadIdx.x == 0) atomicOr(ctr + 2, 1u);
+            }
+            if (pl.ok) {
+                uint32_t done = 0;  // bit kL: carried point
+                for (int iz = 0; iz < pl.nz; ++iz)
+                    for (int iy = 0; iy < pl.ny; ++iy) {
+                        const Win cw = chunk_of(box, pl, iy, iz);
+                        const int cells = cw.ex * cw.ey * cw.ez;
+                        for (int c = threadIdx.x; c < cells * kSlots; c += kW) win[c] = make_int4(0, 0, 0, 0);
+                        __syncthreads();
+                        if (active) {
+                            if (f >= 2 && !((done >> kL) & 1u)) {
+                                const f3 q = grid_coord(cp, origin, a.inv_h);
+                                const Rng rg = point_range(q, G);
+                                if (!rng_empty(rg) && in_win(rg, cw)) {
+                                    if (rng_full(rg))
+                                        splat_block(win, cw, rg, q, cv, slot);
+                                    else
+                                        splat_any(win, cw, true, slot, a.gw.overflow, rg, q, cv, G);
+                                    done |= 1u << kL;
+                                }
+                            }
+#pragma unroll
+                            for (uint32_t t = 0; t < kL; ++t) {
+                                if (!(t + 1 < cnt || (t + 1 == cnt && last_seg))) continue;
+                                if ((done >> t) & 1u) continue;
+                                // opaque copy: keeps the per-point voxel math inside the
+                                // chunk loop instead of hoisting 16 points' worth of it
+                                f3 xp = X[t];
+                                asm volatile("" : "+v"(xp.x), "+v"(xp.y), "+v"(xp.z));
+                                const f3 q = grid_coord(xp, origin, a.inv_h);
+                                const Rng rg = point_range(q, G);
+                                if (rng_empty(rg) || !in_win(rg, cw)) continue;
+                                if (rng_full(rg))
+                                    splat_block(win, cw, rg, q, U[t], slot);
+                                else
+                                    splat_any(win, cw, true, slot, a.gw.overflow, rg, q, U[t], G);
+                                done |= 1u << t;
+                            }
+                        }
+                        __syncthreads();
+                        if (threadIdx.x == 0) {
+                            const uint32_t off = atomicAdd(ctr + 0, (uint32_t)cells);
+                            const uint32_t di = atomicAdd(ctr + 1, 1u);
+                            const bool fits = (uint64_t)off + cells <= a.gw.pool_cap && di < a.gw.desc_cap;
+                            if (di < a.gw.desc_cap) {
+                                int* d = a.gw.desc + ((size_t)a.parity * a.gw.desc_cap + di) * kDescInts;
+                                d[0] = cw.x0;
+                                d[1] = cw.y0;
+                                d[2] = cw.z0;
+                                d[3] = fits ? cw.ex : 0;  // ex == 0: empty descriptor
+                                d[4] = cw.ey;
+                                d[5] = cw.ez;
+                                d[6] = (int)off;
+                                d[7] = 0;
+                            }
+                            if (!fits) atomicOr(ctr + 2, 1u);
+                            bcast[0] = off;
+                            bcast[1] = fits ? 1u : 0u;
+                        }
+                        __syncthreads();
+                        const uint32_t off = bcast[0];
+                        const bool fits = bcast[1] != 0;
+                        int4* dst = a.gw.pool + (size_t)a.parity * a.gw.pool_cap + off;
+                        for (int c = threadIdx.x; c < cells; c += kW) {
+                            // sum the voxel's copies: (u64 vx<<32 | density), vy, vz
+                            uint64_t A = 0;
+                            uint32_t sy = 0, sz = 0;
+#pragma unroll
+                            for (int k = 0; k < kSlots; ++k) {
+                                const int4 raw = win[c * kSlots + k];
+                                A += ((uint64_t)(uint32_t)raw.y << 32) | (uint32_t)raw.x;
+                                sy += (uint32_t)raw.z;
+                                sz += (uint32_t)raw.w;
+                            }
+                            const int4 cell = make_int4((int)(uint32_t)(A >> 32), (int)sy, (int)sz, (int)(uint32_t)A);
+                            if (fits) {
+                                dst[c] = cell;
+                            } else if (cell.x | cell.y | cell.z | cell.w) {
+                                const int ax = c % cw.ex;
+                                const int rest = c / cw.ex;
+                                const size_t gi = (size_t)(cw.x0 + ax) + (size_t)(cw.y0 + rest % cw.ey) * G +
+                                                  (size_t)(cw.z0 + rest / cw.ey) * G * G;
+                                int* gcell = reinterpret_cast<int*>(a.gw.overflow + gi);
+                                atomicAdd(gcell + 0, cell.x);
+                                atomicAdd(gcell + 1, cell.y);
+                                atomicAdd(gcell + 2, cell.z);
+                                atomicAdd(gcell + 3, cell.w);
+                            }
+                        }
+                        __syncthreads();
+                    }
+            }
+        }
+        // carry the segment's last point into the next segment
+#pragma unroll
+        for (uint32_t t = 0; t < kL; ++t)
+            if (t + 1 == cnt) {
+                cp = X[t];
+                cv = U[t];
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reduce: partial boxes -> grid (GridCell ints + float cells); clears the grid.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kReduceThreads = 256;
+constexpr uint32_t kListCap = 4096;
+
+__global__ __launch_bounds__(kReduceThreads) void k_reduce(GridWork gw, uint32_t G, uint32_t parity)
+{
+    constexpr int kCells = kBrick * kBrick * kBrick;
+    constexpr int kRWaves = kReduceThreads / 64;
+    __shared__ int list[kListCap];
+    __shared__ uint32_t nlist;
+    __shared__ uint32_t acc[kCells][4];
+    const uint32_t nb = (G + kBrick - 1) / kBrick;
+    const uint32_t bx = blockIdx.x % nb, by = (blockIdx.x / nb) % nb, bz = blockIdx.x / (nb * nb);
+    const int x0 = bx * kBrick, y0 = by * kBrick, z0 = bz * kBrick;
+    const int x1 = min(x0 + (int)kBrick, (int)G) - 1, y1 = min(y0 + (int)kBrick, (int)G) - 1,
+              z1 = min(z0 + (int)kBrick, (int)G) - 1;
+    const uint32_t* ctr = gw.counters + parity * 4;
+    const uint32_t ndesc = min(ctr[1], gw.desc_cap);
+    const int* desc = gw.desc + (size_t)parity * gw.desc_cap * kDescInts;
+    const int4* pool = gw.pool + (size_t)parity * gw.pool_cap;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+
+    for (int c = threadIdx.x; c < kCells; c += kReduceThreads) {
+        acc[c][0] = 0;
+        acc[c][1] = 0;
+        acc[c][2] = 0;
+        acc[c][3] = 0;
+    }
+    for (uint32_t base = 0; base < ndesc; base += kListCap) {
+        if (threadIdx.x == 0) nlist = 0;
+        __syncthreads();
+        const uint32_t end = min(base + kListCap, ndesc);
+        for (uint32_t i = base + threadIdx.x; i < end; i += kReduceThreads) {
+            const int* d = desc + (size_t)i * kDescInts;
+            const int dx1 = d[0] + d[3] - 1, dy1 = d[1] + d[4] - 1, dz1 = d[2] + d[5] - 1;
+            if (d[3] > 0 && d[0] <= x1 && dx1 >= x0 && d[1] <= y1 && dy1 >= y0 && d[2] <= z1 && dz1 >= z0)
+                list[atomicAdd(&nlist, 1u)] = (int)i;
+        }
+        __syncthreads();
+        const uint32_t n = nlist;
+        // each wavefront takes whole partial boxes; each lane loads up to 8 of
+        // the box-brick intersection's cells at once, then adds them in LDS
+        for (uint32_t j = wave; j < n; j += kRWaves) {
+            const int* d = desc + (size_t)list[j] * kDescInts;
+            const int dx = d[0], dy = d[1], dz = d[2], ex = d[3], ey = d[4], off = d[6];
+            const int ix0 = max(dx, x0), iy0 = max(dy, y0), iz0 = max(dz, z0);
+            const int nx = min(dx + ex - 1, x1) - ix0 + 1;
+            const int ny = min(dy + ey - 1, y1) - iy0 + 1;
+            const int nz = min(dz + d[5] - 1, z1) - iz0 + 1;
+            const int ncell = nx * ny * nz;
+            int4 v[kCells / 64];
+            int li[kCells / 64];
+#pragma unroll
+            for (int k = 0; k < kCells / 64; ++k) {
+                const int c = lane + 64 * k;
+                li[k] = -1;
+                v[k] = make_int4(0, 0, 0, 0);
+                if (c < ncell) {
+                    const int cx = ix0 + c % nx, cy = iy0 + (c / nx) % ny, cz = iz0 + c / (nx * ny);
+                    v[k] = pool[(size_t)off + ((size_t)(cz - dz) * ey + (cy - dy)) * ex + (cx - dx)];
+                    li[k] = ((cz - z0) * kBrick + (cy - y0)) * kBrick + (cx - x0);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kCells / 64; ++k) {
+                if (li[k] < 0) continue;
+                atomicAdd(&acc[li[k]][0], (uint32_t)v[k].x);
+                atomicAdd(&acc[li[k]][1], (uint32_t)v[k].y);
+                atomicAdd(&acc[li[k]][2], (uint32_t)v[k].z);
+                atomicAdd(&acc[li[k]][3], (uint32_t)v[k].w);
+            }
+        }
+        __syncthreads();
+    }
+    const bool ovf = ctr[2] != 0;
+    for (int c = threadIdx.x; c < kCells; c += kReduceThreads) {
+        const int cx = x0 + c % kBrick, cy = y0 + (c / kBrick) % kBrick, cz = z0 + c / (kBrick * kBrick);
+        if (cx > x1 || cy > y1 || cz > z1) continue;
+        const size_t gi = (size_t)cx + (size_t)cy * G + (size_t)cz * G * G;
+        int4 v = make_int4((int)acc[c][0], (int)acc[c][1], (int)acc[c][2], (int)acc[c][3]);
+        if (ovf) {
+            const int4 o = gw.overflow[gi];
+            v.x = (int)((uint32_t)v.x + (uint32_t)o.x);
+            v.y = (int)((uint32_t)v.y + (uint32_t)o.y);
+            v.z = (int)((uint32_t)v.z + (uint32_t)o.z);
+            v.w = (int)((uint32_t)v.w + (uint32_t)o.w);
+            gw.overflow[gi] = make_int4(0, 0, 0, 0);
+        }
+        gw.grid_i[gi] = v;
+        gw.grid_f[gi] = make_float4((float)v.x, (float)v.y, (float)v.z, v.w > 0 ? 1.0f / (float)v.w : 0.0f);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        uint32_t* nxt = gw.counters + (parity ^ 1u) * 4;
+        nxt[0] = 0;
+        nxt[1] = 0;
+        nxt[2] = 0;
     }
 }
 
 // ---------------------------------------------------------------------------
 // AoS <-> SoA (reference Strand layout at the boundary)
 // ---------------------------------------------------------------------------
-// One thread per (strand, point).
-__global__ void k_pack(const float4* __restrict__ aos, uint32_t S, uint32_t N, uint32_t Sp,
-                       float* __restrict__ pos, float* __restrict__ vel, float* __restrict__ corr)
+// One thread per (physical strand slot, point).
+__global__ void k_pack(const float4* __restrict__ aos, const uint32_t* __restrict__ perm, uint32_t S, uint32_t N,
+                       uint32_t Sp, float* __restrict__ pos, float* __restrict__ vel, float* __restrict__ corr)
 {
     const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= (size_t)S * N) return;
-    const uint32_t s = (uint32_t)(tid / N);
+    const uint32_t p = (uint32_t)(tid / N);
     const uint32_t i = (uint32_t)(tid % N);
+    const uint32_t s = perm[p];
     const size_t plane = (size_t)N * Sp;
     const size_t base = (size_t)s * 3 * N;
-    const float4 p = aos[base + i];
-    const float4 v = aos[base + N + i];
-    const size_t o = (size_t)i * Sp + s;
-    pos[o] = p.x;
-    pos[plane + o] = p.y;
-    pos[2 * plane + o] = p.z;
-    vel[o] = v.x;
-    vel[plane + o] = v.y;
-    vel[2 * plane + o] = v.z;
+    const float4 pv = aos[base + i];
+    const float4 vv = aos[base + N + i];
+    const size_t o = (size_t)i * Sp + p;
+    pos[o] = pv.x;
+    pos[plane + o] = pv.y;
+    pos[2 * plane + o] = pv.z;
+    vel[o] = vv.x;
+    vel[plane + o] = vv.y;
+    vel[2 * plane + o] = vv.z;
     if (corr) {
         const float4 c = aos[base + 2 * N + i];
         corr[o] = c.x;
@@ -248,25 +884,29 @@ __global__ void k_pack(const float4* __restrict__ aos, uint32_t S, uint32_t N, u
 
 // SoA -> AoS; when `g` is given, applies the pending grid friction to
 // velocities of points 1..N-1 (compute.comp:257-298) on the way out.
-__global__ void k_unpack(float4* __restrict__ aos, uint32_t S, uint32_t N, uint32_t Sp,
-                         const float* __restrict__ pos, const float* __restrict__ vel,
-                         const float* __restrict__ corr, const int4* __restrict__ g, uint32_t G,
-                         float ox, float oy, float oz, float inv_h, float mu, float omu)
+__global__ void k_unpack(float4* __restrict__ aos, const uint32_t* __restrict__ perm, uint32_t S, uint32_t N,
+                         uint32_t Sp, const float* __restrict__ pos, const float* __restrict__ vel,
+                         const float* __restrict__ corr, const int4* __restrict__ g, uint32_t G, float ox, float oy,
+                         float oz, float inv_h, float mu, float omu)
 {
     const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= (size_t)S * N) return;
-    const uint32_t s = (uint32_t)(tid / N);
+    const uint32_t p = (uint32_t)(tid / N);
     const uint32_t i = (uint32_t)(tid % N);
+    const uint32_t s = perm[p];
     const size_t plane = (size_t)N * Sp;
     const size_t base = (size_t)s * 3 * N;
-    const size_t o = (size_t)i * Sp + s;
-    const f3 p = mk(pos[o], pos[plane + o], pos[2 * plane + o]);
+    const size_t o = (size_t)i * Sp + p;
+    const f3 pp = mk(pos[o], pos[plane + o], pos[2 * plane + o]);
     f3 v = mk(vel[o], vel[plane + o], vel[2 * plane + o]);
     if (g && i >= 1) {
-        const f3 gv = grid_gather(g, grid_coord(p, mk(ox, oy, oz), inv_h), (int)G);
+        const f3 q = grid_coord(pp, mk(ox, oy, oz), inv_h);
+        const Rng rg = point_range(q, (int)G);
+        f3 gv = mk(0.0f, 0.0f, 0.0f);
+        if (!rng_empty(rg)) gv = gather_global_i(g, rg, q, (int)G);
         v = friction(v, gv, mu, omu);
     }
-    aos[base + i] = make_float4(p.x, p.y, p.z, 1.0f);
+    aos[base + i] = make_float4(pp.x, pp.y, pp.z, 1.0f);
     aos[base + N + i] = make_float4(v.x, v.y, v.z, 0.0f);
     float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (corr) c = make_float4(corr[o], corr[plane + o], corr[2 * plane + o], 0.0f);
@@ -276,48 +916,65 @@ __global__ void k_unpack(float4* __restrict__ aos, uint32_t S, uint32_t N, uint3
 // ---------------------------------------------------------------------------
 // Host-side launchers
 // ---------------------------------------------------------------------------
+template <bool G, bool S>
+static void launch_step_e(const StepArgs& a, bool ell, uint32_t blocks, hipStream_t st)
+{
+    if (ell)
+        hipLaunchKernelGGL((k_step<G, S, true>), dim3(blocks), dim3(kW), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_step<G, S, false>), dim3(blocks), dim3(kW), 0, st, a);
+}
+
 hipError_t launch_step(const StepArgs& a, bool gather, bool splat, hipStream_t st)
 {
-    const uint32_t blocks = (a.Sp + kStepBlock - 1) / kStepBlock;
+    const uint32_t blocks = ((a.nwg + 7) / 8) * 8;
+    const bool ell = a.ncol > 1;
     if (gather) {
         if (splat)
-            hipLaunchKernelGGL((k_step<true, true>), dim3(blocks), dim3(kStepBlock), 0, st, a);
+            launch_step_e<true, true>(a, ell, blocks, st);
         else
-            hipLaunchKernelGGL((k_step<true, false>), dim3(blocks), dim3(kStepBlock), 0, st, a);
+            launch_step_e<true, false>(a, ell, blocks, st);
     } else {
         if (splat)
-            hipLaunchKernelGGL((k_step<false, true>), dim3(blocks), dim3(kStepBlock), 0, st, a);
+            launch_step_e<false, true>(a, ell, blocks, st);
         else
-            hipLaunchKernelGGL((k_step<false, false>), dim3(blocks), dim3(kStepBlock), 0, st, a);
+            launch_step_e<false, false>(a, ell, blocks, st);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_pack(const float* aos_dev, uint32_t S, uint32_t N, uint32_t Sp, float* pos,
+hipError_t launch_reduce(const GridWork& gw, uint32_t G, uint32_t parity, hipStream_t st)
+{
+    const uint32_t nb = (G + kBrick - 1) / kBrick;
+    hipLaunchKernelGGL(k_reduce, dim3(nb * nb * nb), dim3(kReduceThreads), 0, st, gw, G, parity);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack(const float* aos_dev, const uint32_t* perm, uint32_t S, uint32_t N, uint32_t Sp, float* pos,
                        float* vel, float* corr, hipStream_t st)
 {
     const size_t n = (size_t)S * N;
     const uint32_t bs = 256;
     const size_t blocks = (n + bs - 1) / bs;
-    hipLaunchKernelGGL(k_pack, dim3((uint32_t)blocks), dim3(bs), 0, st,
-                       reinterpret_cast<const float4*>(aos_dev), S, N, Sp, pos, vel, corr);
+    hipLaunchKernelGGL(k_pack, dim3((uint32_t)blocks), dim3(bs), 0, st, reinterpret_cast<const float4*>(aos_dev),
+                       perm, S, N, Sp, pos, vel, corr);
     return hipGetLastError();
 }
 
-hipError_t launch_unpack(float* aos_dev, uint32_t S, uint32_t N, uint32_t Sp, const float* pos,
-                         const float* vel, const float* corr, const UnpackGrid* g, hipStream_t st)
+hipError_t launch_unpack(float* aos_dev, const uint32_t* perm, uint32_t S, uint32_t N, uint32_t Sp,
+                         const float* pos, const float* vel, const float* corr, const UnpackGrid* g, hipStream_t st)
 {
     const size_t n = (size_t)S * N;
     const uint32_t bs = 256;
     const size_t blocks = (n + bs - 1) / bs;
     if (g)
-        hipLaunchKernelGGL(k_unpack, dim3((uint32_t)blocks), dim3(bs), 0, st,
-                           reinterpret_cast<float4*>(aos_dev), S, N, Sp, pos, vel, corr, g->grid, g->G,
-                           g->ox, g->oy, g->oz, g->inv_h, g->mu, g->omu);
+        hipLaunchKernelGGL(k_unpack, dim3((uint32_t)blocks), dim3(bs), 0, st, reinterpret_cast<float4*>(aos_dev),
+                           perm, S, N, Sp, pos, vel, corr, g->grid, g->G, g->ox, g->oy, g->oz, g->inv_h, g->mu,
+                           g->omu);
     else
-        hipLaunchKernelGGL(k_unpack, dim3((uint32_t)blocks), dim3(bs), 0, st,
-                           reinterpret_cast<float4*>(aos_dev), S, N, Sp, pos, vel, corr,
-                           (const int4*)nullptr, 0u, 0.f, 0.f, 0.f, 0.f, 0.f, 1.f);
+        hipLaunchKernelGGL(k_unpack, dim3((uint32_t)blocks), dim3(bs), 0, st, reinterpret_cast<float4*>(aos_dev),
+                           perm, S, N, Sp, pos, vel, corr, (const int4*)nullptr, 0u, 0.f, 0.f, 0.f, 0.f, 0.f,
+                           1.f);
     return hipGetLastError();
 }
 

@@ -1,0 +1,139 @@
+"""CPU tests of the host side: the C ABI library loads and exports what the
+header declares, the collider matrix helpers (src/Scene.h:28-38,
+src/Scene.cpp:110-136), parameter defaults, workload generators, bench.py's
+command-line contract.  No GPU call is made here."""
+import math
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import hairsim
+from hairsim import scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "hairsim.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(hs_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = header_symbols()
+    assert len(names) >= 19
+    L = hairsim.lib()
+    for n in names:
+        assert hasattr(L, n), n
+    # and the binding covers the same set
+    assert set(hairsim.declared_symbols()) == set(names)
+    out = subprocess.run(["nm", "-D", "--defined-only", hairsim.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (hs_[a-z0-9_]+)", out))
+    assert set(names) <= exported
+
+
+def test_boundary_struct_sizes():
+    import ctypes as C
+    assert C.sizeof(hairsim.HsTime) == 8          # Scene.h:16-19
+    assert C.sizeof(hairsim.HsCollider) == 192    # Scene.h:23-39
+    assert C.sizeof(hairsim.HsGridCell) == 16     # Scene.h:42-49
+    assert C.sizeof(hairsim.HsDrawIndirect) == 16  # Strand.h:53-58
+
+
+def mat(c, which):
+    a = c.as_array()
+    return a[16 * which:16 * which + 16].reshape(4, 4).T.astype(np.float64)  # column-major -> row-major
+
+
+def test_collider_trs_closed_forms():
+    c = hairsim.collider_from_trs((1.0, 2.0, 3.0))
+    T, inv, invT = mat(c, 0), mat(c, 1), mat(c, 2)
+    assert np.allclose(T, [[1, 0, 0, 1], [0, 1, 0, 2], [0, 0, 1, 3], [0, 0, 0, 1]])
+    assert np.allclose(inv @ T, np.eye(4), atol=1e-6)
+    assert np.allclose(invT, inv.T)
+    # rotation about z by 90 degrees maps x to y
+    c = hairsim.collider_from_trs((0, 0, 0), (0, 0, 90), (1, 1, 1))
+    assert np.allclose(mat(c, 0) @ [1, 0, 0, 1], [0, 1, 0, 1], atol=1e-6)
+    # rotation about x by 90 degrees maps y to z; about y maps z to x
+    c = hairsim.collider_from_trs((0, 0, 0), (90, 0, 0), (1, 1, 1))
+    assert np.allclose(mat(c, 0) @ [0, 1, 0, 1], [0, 0, 1, 1], atol=1e-6)
+    c = hairsim.collider_from_trs((0, 0, 0), (0, 90, 0), (1, 1, 1))
+    assert np.allclose(mat(c, 0) @ [0, 0, 1, 1], [1, 0, 0, 1], atol=1e-6)
+    # T * Rz * Ry * Rx * S order: scale first, translate last
+    c = hairsim.collider_from_trs((5, 0, 0), (0, 0, 90), (2, 1, 1))
+    assert np.allclose(mat(c, 0) @ [1, 0, 0, 1], [5, 2, 0, 1], atol=1e-6)
+
+
+def test_reference_demo_colliders_invert():
+    spec = [((0.0, 2.64, 0.08), (-38.270, 0, 0), (0.817, 1.158, 1.01)),
+            ((-0.698, 0.087, -0.36), (-20.254, 13.144, 34.5), (0.721, 1.0, 0.724))]
+    for s in spec:
+        c = hairsim.collider_from_trs(*s)
+        T, inv, invT = mat(c, 0), mat(c, 1), mat(c, 2)
+        assert np.allclose(inv @ T, np.eye(4), atol=1e-5)
+        assert np.array_equal(invT, inv.T)
+
+
+def test_collider_translate_is_translate_sphere():
+    """Scene::translateSphere: transform = transform * translate(t)."""
+    c = hairsim.collider_from_trs((1.0, 0.0, 0.0), (0, 0, 0), (2, 2, 2))
+    hairsim.collider_translate(c, (0.5, 0.0, -1.0))
+    T = mat(c, 0)
+    assert np.allclose(T[:3, 3], [1.0 + 2 * 0.5, 0.0, -2.0])
+    assert np.allclose(mat(c, 1) @ T, np.eye(4), atol=1e-6)
+
+
+def test_reference_defaults():
+    p = hairsim.reference_step_params()
+    assert list(p.gravity) == [0.0, np.float32(-9.8), 0.0]
+    assert p.damping == np.float32(0.998) and p.spring_k == 1900.0 and p.sphere_radius == 1.0
+    assert p.vmax == 10.0 and p.friction == np.float32(0.08) and p.num_colliders == 6 and p.hair_hair == 1
+    d = hairsim.HsGroomDesc()
+    hairsim.lib().hs_groom_desc_reference(d, 900, 10)
+    assert d.num_strands == 900 and d.num_points == 10 and d.strand_length == 2.5
+    assert d.grid_dim == 64 and d.grid_h == np.float32(7.0 / 64.0)
+    assert list(d.grid_origin) == [-3.0, -2.0, -5.0]
+
+
+def test_groom_bytes():
+    b1 = hairsim.groom_bytes(100_000, 32, 64)
+    b0 = hairsim.groom_bytes(100_000, 32, 0)
+    assert b1 > b0 > 2 * 3 * 32 * 100_000 * 4
+    assert hairsim.groom_bytes(0, 32, 64) == 0
+
+
+def test_hemisphere_workloads():
+    wl = scenes.plain_1k()
+    g = wl.grooms[0]
+    P = g.strands.reshape(-1, 3, g.N, 4)[:, 0]
+    roots = P[:, 0, :3].astype(np.float64)
+    assert np.allclose(np.linalg.norm(roots, axis=1), 1.0, atol=1e-6)
+    assert (roots[:, 1] >= 0).all()
+    seg = np.linalg.norm(np.diff(P[:, :, :3].astype(np.float64), axis=1), axis=2)
+    assert np.allclose(seg, 1.0 / 15.0, atol=1e-6)
+    assert np.all(P[:, :, 3] == 1.0)
+    assert np.all(g.strands.reshape(-1, 3, g.N, 4)[:, 1:] == 0.0)
+    w = scenes.plain_100k(S=2000)
+    g = w.grooms[0]
+    L = g.seg.astype(np.float64) * 31
+    assert L.min() >= 0.8 - 1e-6 and L.max() <= 1.2 + 1e-6
+    assert abs(float(np.std(g.noise)) - 0.1) < 0.01
+    assert w.wind(14)[0] == pytest.approx(2.0 * math.sin(2 * math.pi * 15 / 60), rel=1e-5)
+    c = w.sphere(0)
+    assert math.hypot(c[0], c[2]) == pytest.approx(1.1)
+
+
+def test_algorithmic_bytes_formula():
+    assert scenes.algorithmic_bytes(100_000, 32, 64, True) == 100_000 * (48 * 31 + 16) + 32 * 64 ** 3
+    assert scenes.algorithmic_bytes(1024, 16, 0, False) == 1024 * (48 * 15 + 16)
+
+
+def test_bench_rejects_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr

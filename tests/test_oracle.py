@@ -1,0 +1,372 @@
+"""Pins of the CPU oracle (oracle/libhair_oracle.so) that do not restate its code.
+
+The reference ships no tests, fixtures or golden vectors for the physics pass,
+so each stage of compute.comp is pinned here by what the mathematics fixes:
+closed forms, invariants, brute force on tiny inputs and relations between the
+outputs, chosen so that a dropped term, a wrong sign or index, or a transposed
+operand fails at least one of them.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from hairsim import scenes
+
+G = 64
+
+
+def grid_params(**kw):
+    args = dict(dt=1.0 / 60.0, hair_hair=True, grid_dim=G, grid_origin=(0.0, 0.0, 0.0), grid_h=1.0)
+    args.update(kw)
+    return oracle.Params(**args)
+
+
+def cells_of(grid):
+    g = grid.reshape(-1, 4)
+    nz = np.flatnonzero(np.any(g != 0, axis=1))
+    return {int(i): tuple(int(v) for v in g[i]) for i in nz}
+
+
+# ---------------------------------------------------------------- int() --
+@pytest.mark.parametrize("f,expect", [
+    (2.9, 2), (-2.9, -2), (0.0, 0), (-0.0, 0), (float("nan"), 0),
+    (float("inf"), 2**31 - 1), (-float("inf"), -2**31), (2147483648.0, 2**31 - 1),
+    (-2147483648.0, -2**31), (2147483520.0, 2147483520), (-1e12, -2**31), (1e12, 2**31 - 1)])
+def test_saturating_int(f, expect):
+    assert oracle.sat_i32(f) == expect
+
+
+# ---------------------------------------------------------------- splat --
+def test_splat_single_point_dyadic_weights():
+    """q = (10.25, 20.5, 30.75): trilinear weights are dyadic, so every int is exact."""
+    p = grid_params()
+    grid = p.grid_buffer()
+    v = np.array([1.5, -2.25, 0.5])
+    oracle.splat32(p, [10.25, 20.5, 30.75], v, grid)
+    wx = {10: 0.75, 11: 0.25}
+    wy = {20: 0.5, 21: 0.5}
+    wz = {30: 0.25, 31: 0.75}
+    expect = {}
+    for a, wa in wx.items():
+        for b, wb in wy.items():
+            for c, wc in wz.items():
+                w = wa * wb * wc
+                expect[a + b * G + c * G * G] = (int(1e6 * w * v[0]), int(1e6 * w * v[1]), int(1e6 * w * v[2]),
+                                                 int(1e6 * w))
+    assert cells_of(grid) == expect
+    assert sum(c[3] for c in expect.values()) == 1_000_000
+
+
+def test_splat_edges():
+    """Below 0 only cell 0 is touched (weight 1-|q|); at or beyond G nothing is; just below G only G-1."""
+    p = grid_params()
+    grid = p.grid_buffer()
+    oracle.splat32(p, [-0.25, 5.0, 5.0], [0.0, 0.0, 0.0], grid)
+    got = cells_of(grid)
+    assert set(got) == {0 + 5 * G + 5 * G * G}
+    assert got[5 * G + 5 * G * G][3] == 750000
+    grid[:] = 0
+    oracle.splat32(p, [G + 0.0, 5.0, 5.0], [1.0, 1.0, 1.0], grid)
+    assert cells_of(grid) == {}
+    grid[:] = 0
+    oracle.splat32(p, [-1.0, 5.0, 5.0], [1.0, 1.0, 1.0], grid)
+    assert cells_of(grid) == {}
+    grid[:] = 0
+    oracle.splat32(p, [G - 0.5, 5.0, 5.0], [0.0, 0.0, 0.0], grid)
+    got = cells_of(grid)
+    assert set(got) == {(G - 1) + 5 * G + 5 * G * G}
+    assert got[(G - 1) + 5 * G + 5 * G * G][3] == 500000
+
+
+def test_splat_conservation_random_points():
+    """Interior points: density sums to 1e6 minus at most 8 truncations (+1 for fp rounding)."""
+    rng = np.random.default_rng(3)
+    p = grid_params()
+    for _ in range(200):
+        q = rng.uniform(1.0, G - 2.0, 3)
+        v = rng.normal(0, 3, 3)
+        grid = p.grid_buffer()
+        oracle.splat32(p, q, v, grid)
+        g = grid.reshape(-1, 4).astype(np.int64)
+        d = g[:, 3].sum()
+        assert 1_000_000 - 9 <= d <= 1_000_000
+        mom = g[:, :3].sum(axis=0)
+        assert np.all(np.abs(mom - 1e6 * v) <= 8 + 1e-5 * np.abs(1e6 * v))
+
+
+def test_splat_order_independent_and_mt_matches_st():
+    wl = scenes.plain_100k(S=3000, N=12, G=32)
+    g = wl.grooms[0]
+
+    def params():
+        return oracle.Params(dt=wl.dt, wind=wl.wind(0), damping=wl.damping, sphere_radius=wl.sphere_radius,
+                             friction=wl.friction, hair_hair=True, grid_dim=g.grid_dim, grid_origin=g.grid_origin,
+                             grid_h=g.grid_h, colliders=scenes.sphere_collider(wl.sphere(0)), seg_length=g.seg,
+                             wind_noise=g.noise)
+    a = g.strands.copy()
+    ga = oracle.step32(params(), a, g.N)
+    b = g.strands.copy()
+    gb = oracle.step32_mt(params(), b, g.N, threads=4)
+    assert np.array_equal(ga, gb)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    perm = np.random.default_rng(1).permutation(g.S)
+    c = np.ascontiguousarray(g.strands[perm])
+    pc = oracle.Params(dt=wl.dt, wind=wl.wind(0), damping=wl.damping, sphere_radius=wl.sphere_radius,
+                       friction=wl.friction, hair_hair=True, grid_dim=g.grid_dim, grid_origin=g.grid_origin,
+                       grid_h=g.grid_h, colliders=scenes.sphere_collider(wl.sphere(0)), seg_length=g.seg[perm],
+                       wind_noise=g.noise[perm])
+    gc = oracle.step32(pc, c, g.N)
+    assert np.array_equal(ga, gc)
+    assert np.array_equal(a[perm].view(np.uint32), c.view(np.uint32))
+
+
+# --------------------------------------------------------------- gather --
+def test_gather_uniform_field_recovers_velocity():
+    """Every voxel around the query occupied by points of one velocity v: the gather
+    (sum over the 8 voxels of w * vel/density, weights summing to 1) returns v."""
+    rng = np.random.default_rng(5)
+    p = grid_params()
+    grid = p.grid_buffer()
+    v = np.array([0.7, -1.3, 2.1])
+    for _ in range(3000):
+        oracle.splat32(p, rng.uniform(20, 26, 3), v, grid)
+    for _ in range(20):
+        q = rng.uniform(21, 25, 3)
+        gv = oracle.gather32(p, q, grid)
+        assert np.allclose(gv, v, rtol=0, atol=2e-4), (gv, v)
+        gv64 = oracle.gather64(p, q, grid)
+        assert np.allclose(gv64, v, rtol=0, atol=2e-4)
+
+
+def test_gather_single_point_and_skip_of_empty_cells():
+    p = grid_params()
+    grid = p.grid_buffer()
+    v = np.array([1.0, 2.0, -3.0])
+    oracle.splat32(p, [7.5, 7.5, 7.5], v, grid)
+    assert np.allclose(oracle.gather32(p, [7.5, 7.5, 7.5], grid), v, atol=1e-5)
+    # a cell with density <= 0 is skipped (compute.comp:281)
+    grid[:] = 0
+    grid[4 * (3 + 3 * G + 3 * G * G):][:4] = [1000, 1000, 1000, 0]
+    assert np.array_equal(oracle.gather32(p, [3.0, 3.0, 3.0], grid), np.zeros(3, np.float32))
+    grid[4 * (3 + 3 * G + 3 * G * G) + 3] = -5
+    assert np.array_equal(oracle.gather32(p, [3.0, 3.0, 3.0], grid), np.zeros(3, np.float32))
+
+
+# --------------------------------------------------------------- forces --
+def sphere(center):
+    return scenes.sphere_collider(center)
+
+
+def test_force_gravity_wind_noise():
+    p = oracle.Params(dt=1 / 60, gravity=(0.0, -9.8, 0.0), wind=(1.5, 0.0, -0.5), colliders=None,
+                      wind_noise=np.zeros(1, np.float32), noise_dir=(0.0, 0.0, 2.0))
+    hits, f = oracle.force64(p, [5.0, 5.0, 5.0], 0.25)
+    assert hits == 0
+    assert np.allclose(f, [1.5, -9.8, -0.5 + 0.5], atol=1e-6)
+
+
+def test_force_sphere_closed_form():
+    """Inside at depth r - delta: F = g + k (r - delta) n; outside: F = g (binary32-exact constants)."""
+    c = np.array([0.0, 0.0, 1.25])
+    g = np.array([0.0, -8.0, 0.0])
+    p = oracle.Params(dt=1 / 64, gravity=tuple(g), sphere_radius=0.375, spring_k=1900.0, colliders=sphere(c))
+    rng = np.random.default_rng(2)
+    for _ in range(50):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        delta = rng.uniform(0.01, 0.37)
+        hits, f = oracle.force64(p, c + delta * n, 0.0)
+        assert hits == 1
+        assert np.allclose(f, g + 1900.0 * (0.375 - delta) * n, rtol=1e-9, atol=1e-9)
+        hits, f = oracle.force64(p, c + 0.38 * n, 0.0)
+        assert hits == 0 and np.array_equal(f, g)
+
+
+def ellipsoid(center, scale):
+    T = np.diag([scale[0], scale[1], scale[2], 1.0])
+    T[:3, 3] = center
+    inv = np.linalg.inv(T)
+    return np.concatenate([T.T.reshape(-1), inv.T.reshape(-1), inv.reshape(-1)]).astype(np.float32)
+
+
+def test_force_ellipsoid_unit_scale_is_unit_sphere():
+    """An ellipsoid collider with unit scale and no rotation acts as a unit sphere at its centre."""
+    c = np.array([0.5, 2.0, -1.0])
+    cols = np.stack([sphere([100.0, 100.0, 100.0]), ellipsoid(c, (1, 1, 1))])
+    p = oracle.Params(dt=1 / 60, colliders=cols)
+    rng = np.random.default_rng(4)
+    for _ in range(30):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        delta = rng.uniform(0.05, 0.95)
+        hits, f = oracle.force64(p, c + delta * n, 0.0)
+        assert hits == 1
+        assert np.allclose(f, np.array([0, -9.8, 0]) + 1900.0 * (1 - delta) * n, rtol=1e-6, atol=1e-6)
+
+
+def test_force_ellipsoid_axis_scaled_and_average():
+    c = np.array([0.0, 0.0, 0.0])
+    cols = np.stack([sphere([0.0, 0.0, 0.0]), ellipsoid(c, (2, 1, 1))])
+    p = oracle.Params(dt=1 / 64, gravity=(0.0, -8.0, 0.0), sphere_radius=1.5, colliders=cols)
+    # on the x axis at 1: inside the ellipsoid (x/2 = 0.5); surface point (2,0,0): depth 1, normal +x.
+    # the sphere (r 1.5) is hit too with depth 0.5, normal +x: force = g + (1900*1 + 1900*0.5)/2 x
+    hits, f = oracle.force64(p, [1.0, 0.0, 0.0], 0.0)
+    assert hits == 2
+    assert np.allclose(f, [1900.0 * 1.5 / 2, -8.0, 0.0], rtol=1e-12)
+
+
+# ----------------------------------------------------- predict + FTL ----
+def test_single_segment_closed_form():
+    """N = 2: the free point moves to the projection of its prediction on the sphere |p - root| = r."""
+    dt, r, D = 0.125, 0.5, 0.875   # binary32-exact
+    st = np.zeros((1, 24), np.float64)
+    st[0, 4:7] = [r, 0.0, 0.0]
+    p = oracle.Params(dt=dt, damping=D, gravity=(0.0, -8.0, 0.0), seg_length=np.array([r], np.float32))
+    x = np.array([r, 0.0, 0.0])
+    oracle.step64(p, st, 2)
+    drop = dt * dt * 8.0
+    th = math.atan2(drop, r)
+    pn = np.array([r * math.cos(th), -r * math.sin(th), 0.0])
+    assert np.allclose(st[0, 4:7], pn, rtol=0, atol=1e-15)
+    assert np.allclose(st[0, 12:15], D * (pn - x) / dt, rtol=0, atol=1e-13)  # vel of point 1 (N=2)
+    assert np.allclose(st[0, :3], 0.0) and st[0, 0:3].tolist() == [0.0, 0.0, 0.0]
+
+
+def random_strands(rng, S, N, r):
+    roots = rng.normal(size=(S, 3))
+    st = np.zeros((S, 3, N, 4))
+    for s in range(S):
+        p = roots[s].copy()
+        for i in range(N):
+            st[s, 0, i, :3] = p
+            st[s, 0, i, 3] = 1.0
+            d = rng.normal(size=3)
+            p = p + r * d / np.linalg.norm(d)
+        st[s, 1, :, :3] = rng.normal(scale=0.5, size=(N, 3))
+    return st.reshape(S, 12 * N)
+
+
+def test_ftl_invariants_fp64_and_fp32():
+    rng = np.random.default_rng(9)
+    S, N, r = 64, 12, 0.07
+    st = random_strands(rng, S, N, r)
+    seg = np.full(S, r, np.float32)
+    p = oracle.Params(dt=1 / 60, damping=0.9, seg_length=seg, sphere_radius=0.3, colliders=sphere([0, 0, 0]))
+    before = st.copy()
+    oracle.step64(p, st, N)
+    P0, P1 = before.reshape(S, 3, N, 4)[:, 0, :, :3], st.reshape(S, 3, N, 4)[:, 0, :, :3]
+    assert np.array_equal(P1[:, 0], P0[:, 0])
+    seglen = np.linalg.norm(np.diff(P1, axis=1), axis=2)
+    assert np.max(np.abs(seglen / np.float64(np.float32(r)) - 1)) < 1e-12
+    # FP32: segment length within a few ulp of |p| relative to r
+    st32 = before.astype(np.float32)
+    oracle.step32(p, st32, N)
+    P32 = st32.reshape(S, 3, N, 4)[:, 0, :, :3].astype(np.float64)
+    seg32 = np.linalg.norm(np.diff(P32, axis=1), axis=2)
+    ulp = np.spacing(np.float32(np.abs(P32).max())).astype(np.float64)
+    assert np.max(np.abs(seg32 - np.float32(r))) <= 8 * ulp
+
+
+def test_ftl_point_lies_on_ray_toward_prediction():
+    rng = np.random.default_rng(11)
+    S, N, r, dt = 32, 8, 0.125, 1 / 64
+    st = random_strands(rng, S, N, r)
+    seg = np.full(S, r, np.float32)
+    p = oracle.Params(dt=dt, damping=0.875, gravity=(0.0, -8.0, 0.0), seg_length=seg)
+    before = st.copy().reshape(S, 3, N, 4)
+    oracle.step64(p, st, N)
+    after = st.reshape(S, 3, N, 4)
+    for s in range(S):
+        for i in range(1, N):
+            x, u = before[s, 0, i, :3], before[s, 1, i, :3]
+            pred = x + dt * u + dt * dt * np.array([0, -8.0, 0])
+            par = after[s, 0, i - 1, :3]
+            d1 = after[s, 0, i, :3] - par
+            d2 = pred - par
+            assert np.linalg.norm(np.cross(d1, d2)) <= 1e-12 * np.linalg.norm(d2)
+            assert np.dot(d1, d2) > 0
+
+
+def test_dftl_relation_between_outputs():
+    """Tip: v = D (p_new - x)/dt.  Others: v_i = D (p_new_i - x_i)/dt - c_{i+1}/dt, and
+    c_i = D (p_new_i - pred_i) with pred_i = x_i + dt u_i + dt^2 F(x_i)."""
+    rng = np.random.default_rng(13)
+    S, N, r, dt, D = 16, 9, 0.05, 1 / 60, 0.9
+    st = random_strands(rng, S, N, r) * 0.2
+    seg = np.full(S, r, np.float32)
+    p = oracle.Params(dt=dt, damping=D, vmax=1e9, seg_length=seg, wind=(0.5, 0.0, 0.2))
+    before = st.copy().reshape(S, 3, N, 4)
+    oracle.step64(p, st, N)
+    after = st.reshape(S, 3, N, 4)
+    inv_dt = 1.0 / np.float64(np.float32(dt))
+    dtf = np.float64(np.float32(dt))
+    for s in range(S):
+        X, U = before[s, 0, :, :3], before[s, 1, :, :3]
+        P, V, Cc = after[s, 0, :, :3], after[s, 1, :, :3], after[s, 2, :, :3]
+        for i in range(1, N):
+            pred = X[i] + dtf * U[i] + dtf * dtf * np.array([0.5, np.float32(-9.8), 0.2])
+            assert np.allclose(Cc[i], D * (P[i] - pred), atol=1e-12)
+            base = np.float64(np.float32(D)) * (P[i] - X[i]) * inv_dt
+            if i == N - 1:
+                assert np.allclose(V[i], base, atol=1e-9)
+            else:
+                assert np.allclose(V[i], base - Cc[i + 1] * inv_dt, atol=1e-9)
+        assert np.array_equal(V[0], U[0])
+
+
+def test_velocity_clamp_boundary():
+    """|v| just above vmax is clamped to vmax along v; just below is untouched."""
+    dt, r, D = 1.0, 16.0, 1.0
+    for target, clamped in ((10.000001, True), (9.999999, False)):
+        # root at 0, point at (r,0,0); the prediction points at angle ang so that the FTL
+        # displacement is a chord of length target (|v| = target with dt = D = 1)
+        st = np.zeros((1, 24))
+        st[0, 4:7] = [r, 0.0, 0.0]
+        ang = 2 * math.asin(target / (2 * r))
+        pred_dir = np.array([math.cos(ang), 0.0, math.sin(ang)])
+        u = pred_dir * 20.0 - np.array([r, 0.0, 0.0])
+        st[0, 12:15] = u  # vel of point 1 (N=2: floats 4N + 4)
+        p = oracle.Params(dt=dt, damping=D, gravity=(0.0, 0.0, 0.0), seg_length=np.array([r], np.float32))
+        oracle.step64(p, st, 2)
+        v = st[0, 12:15]
+        if clamped:
+            assert abs(np.linalg.norm(v) - 10.0) < 1e-12
+        else:
+            assert abs(np.linalg.norm(v) - target) < 1e-9
+
+
+def test_hanging_strand_positions_are_fixed():
+    """A straight strand hanging along -g with no colliders keeps its positions (FP64)."""
+    N, r = 10, float(np.float32(0.1))
+    st = np.zeros((1, 3, N, 4))
+    for i in range(N):
+        st[0, 0, i] = [0.3, 2.0 - i * r, -0.2, 1.0]
+    st = st.reshape(1, 12 * N)
+    p = oracle.Params(dt=1 / 60, damping=0.9, seg_length=np.array([r], np.float32))
+    before = st.copy()
+    for _ in range(5):
+        oracle.step64(p, st, N)
+    P0 = before.reshape(1, 3, N, 4)[0, 0, :, :3]
+    P1 = st.reshape(1, 3, N, 4)[0, 0, :, :3]
+    assert np.max(np.abs(P1 - P0)) < 1e-12
+
+
+# ---------------------------------------------------- fp32 vs fp64 ------
+def test_ref32_tracks_oracle64_per_step():
+    """Same FP32 state into both each step: positions within 1e-5 * max(|p|, 1)."""
+    wl = scenes.plain_1k(S=256)
+    g = wl.grooms[0]
+    s32 = g.strands.copy()
+    worst = 0.0
+    for k in range(240):
+        p = oracle.Params(dt=wl.dt, damping=wl.damping, sphere_radius=wl.sphere_radius,
+                          colliders=sphere(wl.sphere(k)), seg_length=g.seg)
+        s64 = s32.astype(np.float64)
+        oracle.step64(p, s64, g.N)
+        oracle.step32(p, s32, g.N)
+        P32 = s32.reshape(-1, 3, g.N, 4)[:, 0, :, :3]
+        P64 = s64.reshape(-1, 3, g.N, 4)[:, 0, :, :3]
+        worst = max(worst, float(np.max(np.abs(P32 - P64) / np.maximum(np.abs(P64), 1.0))))
+    assert worst <= 1e-5, worst
