@@ -187,6 +187,23 @@ int hs_step(hs_ctx* ctx, uint32_t groom_id, const hs_time* time,
             const hs_collider* colliders, uint32_t num_colliders,
             const hs_step_params* params);
 
+/* One step of many grooms at once.  The reference steps each `Hair` with its
+ * own vkCmdDispatch inside the one compute command buffer
+ * (src/Renderer.cpp:2066-2071, where the descriptor index `4 + i` only works
+ * for one groom); here all grooms' strand blocks go to one step launch and
+ * all their voxel bricks to one reduce launch (per group of grooms that need
+ * the same kernel variant).  Each groom's result is identical to stepping it
+ * alone with hs_step.  A groom may appear at most once per call. */
+typedef struct hs_batch_item {
+    uint32_t groom_id;
+    hs_time time;
+    const hs_collider* colliders;
+    uint32_t num_colliders;
+    const hs_step_params* params;
+} hs_batch_item;
+
+int hs_step_batch(hs_ctx* ctx, const hs_batch_item* items, uint32_t num_items);
+
 /* Read the groom back in the reference Strand layout (48*N B per strand) with
  * the velocities after this step's grid friction, plus the indirect draw
  * record {S, 1, 0, 0} the shader's count at compute.comp:302 produces.

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <new>
 #include <numeric>
@@ -100,6 +101,10 @@ struct hs_ctx {
     size_t events_used = 0;
     uint64_t prof_launches = 0;
     double prof_step_ms = 0.0, prof_reduce_ms = 0.0;
+    // device staging of hs_step_batch arguments
+    hs::StepArgs* batch_args = nullptr;
+    uint32_t* batch_idx = nullptr;
+    size_t batch_args_cap = 0, batch_idx_cap = 0;
 };
 
 namespace {
@@ -190,49 +195,159 @@ float sqrt_threshold(float vmax)
     return x;
 }
 
-// Morton order of the roots: strands that start close are stored close, so a
-// workgroup's strands touch a compact set of voxels.
-std::vector<uint32_t> morton_perm(const float* aos, uint32_t S, uint32_t N)
+// Smallest float T with sqrtf(T) >= r, so that for every non-NaN d >= 0,
+// d < T exactly when sqrtf(d) < r.  Lets the kernel test a sphere hit
+// without a square root.
+float sqrt_lower_threshold(float r)
 {
-    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t s = 0; s < S; ++s) {
-        const float* r = aos + (size_t)s * 12 * N;
-        for (int k = 0; k < 3; ++k)
-            if (std::isfinite(r[k])) {
-                lo[k] = std::min(lo[k], r[k]);
-                hi[k] = std::max(hi[k], r[k]);
-            }
-    }
-    auto spread = [](uint32_t v) {
-        v &= 0x3FF;
-        v = (v | (v << 16)) & 0x030000FF;
-        v = (v | (v << 8)) & 0x0300F00F;
-        v = (v | (v << 4)) & 0x030C30C3;
-        v = (v | (v << 2)) & 0x09249249;
-        return v;
-    };
-    std::vector<uint64_t> key(S);
-    for (uint32_t s = 0; s < S; ++s) {
-        const float* r = aos + (size_t)s * 12 * N;
-        uint32_t q[3];
+    if (std::isnan(r)) return -INFINITY;   // sqrt(d) < NaN is never true
+    if (!(r > 0.0f)) return 0.0f;          // sqrt(d) >= 0 >= r
+    if (std::isinf(r)) return INFINITY;
+    float x = r * r;
+    if (std::isinf(x)) x = std::numeric_limits<float>::max();
+    while (x > 0.0f && std::sqrt(std::nextafter(x, 0.0f)) >= r) x = std::nextafter(x, 0.0f);
+    while (std::sqrt(x) < r) x = std::nextafter(x, INFINITY);
+    return x;
+}
+
+// Physical strand order: a balanced k-d split of the roots into leaves of
+// exactly kW strands (one workgroup each; only the last leaf may be short).
+// Every workgroup's strands start close together, so the voxels they touch
+// form a compact box; unlike a space-filling curve there is no jump between
+// distant patches inside one workgroup.
+std::vector<uint32_t> kd_perm(const float* aos, uint32_t S, uint32_t N)
+{
+    std::vector<float> root(3ull * S);
+    for (uint32_t s = 0; s < S; ++s)
         for (int k = 0; k < 3; ++k) {
-            const float ext = hi[k] - lo[k];
-            float t = (ext > 0.0f && std::isfinite(r[k])) ? (r[k] - lo[k]) / ext : 0.0f;
-            t = std::min(std::max(t, 0.0f), 1.0f);
-            q[k] = (uint32_t)(t * 1023.0f);
+            const float v = aos[(size_t)s * 12 * N + k];
+            root[3ull * s + k] = std::isnan(v) ? std::numeric_limits<float>::max() : v;
         }
-        const uint64_t code = spread(q[0]) | (spread(q[1]) << 1) | (spread(q[2]) << 2);
-        key[s] = (code << 32) | s;  // ties broken by index: deterministic
+    std::vector<uint32_t> idx(S);
+    std::iota(idx.begin(), idx.end(), 0u);
+    struct Range {
+        uint32_t b, e;
+    };
+    std::vector<Range> stack{{0, S}};
+    while (!stack.empty()) {
+        const Range r = stack.back();
+        stack.pop_back();
+        const uint32_t n = r.e - r.b;
+        if (n <= hs::kW) continue;
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t i = r.b; i < r.e; ++i)
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = std::min(lo[k], root[3ull * idx[i] + k]);
+                hi[k] = std::max(hi[k], root[3ull * idx[i] + k]);
+            }
+        int ax = 0;
+        for (int k = 1; k < 3; ++k)
+            if (hi[k] - lo[k] > hi[ax] - lo[ax]) ax = k;
+        // split at the multiple of kW nearest the median: the left part is
+        // always whole workgroups, the remainder travels right
+        uint32_t mid = (uint32_t)(((n / 2) + hs::kW / 2) / hs::kW) * hs::kW;
+        mid = std::min(std::max(mid, (uint32_t)hs::kW), n - 1);
+        auto cmp = [&](uint32_t a, uint32_t b) {
+            const float fa = root[3ull * a + ax], fb = root[3ull * b + ax];
+            return fa < fb || (fa == fb && a < b);
+        };
+        std::nth_element(idx.begin() + r.b, idx.begin() + r.b + mid, idx.begin() + r.e, cmp);
+        stack.push_back({r.b + mid, r.e});
+        stack.push_back({r.b, r.b + mid});
     }
-    std::sort(key.begin(), key.end());
-    std::vector<uint32_t> perm(S);
-    for (uint32_t p = 0; p < S; ++p) perm[p] = (uint32_t)(key[p] & 0xFFFFFFFFu);
-    return perm;
+    return idx;
+}
+
+// Validate one step request and fill the kernel arguments for groom g.
+int build_args(hs_ctx* c, Groom& g, const hs_time* t, const hs_collider* cols, uint32_t ncol,
+               const hs_step_params* p, const char* who, hs::StepArgs* out, bool* gather, bool* splat)
+{
+    const std::string w(who);
+    if (!t || !p) return fail(c, HS_ERR_INVALID, w + ": null time or params");
+    if (ncol > HS_MAX_COLLIDERS) return fail(c, HS_ERR_INVALID, w + ": more than 8 colliders");
+    if (ncol > 0 && !cols) return fail(c, HS_ERR_INVALID, w + ": null colliders");
+    if (p->num_colliders > ncol) return fail(c, HS_ERR_INVALID, w + ": params.num_colliders > colliders given");
+    if (!(t->deltaTime > 0.0f) || !std::isfinite(t->deltaTime))
+        return fail(c, HS_ERR_INVALID, w + ": deltaTime must be finite and > 0");
+    if (p->mesh_id != 0) return fail(c, HS_ERR_NOT_FOUND, w + ": unknown mesh");
+    hs::StepArgs& a = *out;
+    std::memset(&a, 0, sizeof(a));
+    a.pos = g.pos;
+    a.vel = g.vel;
+    a.corr = g.corr;
+    a.seg = g.seg;
+    a.noise = g.noise;
+    a.S = g.S;
+    a.N = g.N;
+    a.Sp = g.L.Sp;
+    a.nwg = g.L.nwg;
+    a.nseg = g.L.nseg;
+    a.gw = g.gw;
+    a.G = g.G;
+    a.parity = g.parity;
+    a.ox = g.origin[0];
+    a.oy = g.origin[1];
+    a.oz = g.origin[2];
+    a.inv_h = g.G ? 1.0f / g.h : 0.0f;
+    a.dt = t->deltaTime;
+    a.inv_dt = 1.0f / a.dt;
+    a.dt2 = a.dt * a.dt;
+    a.D = p->damping;
+    a.k = p->spring_k;
+    a.vmax = p->vmax;
+    a.vmax2 = sqrt_threshold(p->vmax);
+    a.sphere_r = p->sphere_radius;
+    a.sphere_r2 = sqrt_lower_threshold(p->sphere_radius);
+    a.fx = p->gravity[0] + p->wind[0];
+    a.fy = p->gravity[1] + p->wind[1];
+    a.fz = p->gravity[2] + p->wind[2];
+    a.ndx = g.noise_dir[0];
+    a.ndy = g.noise_dir[1];
+    a.ndz = g.noise_dir[2];
+    a.ncol = p->num_colliders;
+    a.cols = g.cols;
+    if (a.ncol > 0) {
+        a.sphere_c[0] = cols[0].transform[12];
+        a.sphere_c[1] = cols[0].transform[13];
+        a.sphere_c[2] = cols[0].transform[14];
+    }
+    if (a.ncol > 1) {
+        // ellipsoids are read from device memory; stream order keeps the
+        // previous step's kernel ahead of this copy
+        hs::Collider host[hs::kMaxColliders];
+        for (uint32_t j = 0; j < a.ncol; ++j) {
+            std::memcpy(host[j].m, cols[j].transform, 64);
+            std::memcpy(host[j].m + 16, cols[j].inv, 64);
+            std::memcpy(host[j].m + 32, cols[j].invTrans, 64);
+        }
+        HS_TRY(c, hipMemcpyAsync(g.cols, host, sizeof(hs::Collider) * a.ncol, hipMemcpyHostToDevice, c->stream));
+    }
+    *gather = g.pending;
+    if (*gather) {
+        a.mu_prev = g.pend_mu;
+        a.omu_prev = g.pend_omu;
+    }
+    *splat = p->hair_hair && g.G > 0;
+    return HS_OK;
+}
+
+// Host-side state after a launched step: pending friction and pool parity.
+void advance_state(Groom& g, bool splat, float friction)
+{
+    if (splat) {
+        g.parity ^= 1u;
+        g.pending = true;
+        g.pend_mu = friction;
+        g.pend_omu = 1.0f - friction;
+    } else {
+        g.pending = false;
+    }
 }
 
 }  // namespace
 
 extern "C" {
+
 
 void hs_step_params_reference(hs_step_params* p)
 {
@@ -292,6 +407,8 @@ void hs_destroy(hs_ctx* c)
     c->grooms.clear();
     for (auto& ev : c->events)
         for (int k = 0; k < 3; ++k) (void)hipEventDestroy(ev.e[k]);
+    (void)hipFree(c->batch_args);
+    (void)hipFree(c->batch_idx);
     if (c->own) (void)hipStreamDestroy(c->own);
     delete c;
 }
@@ -336,9 +453,9 @@ int hs_groom_create(hs_ctx* c, const hs_groom_desc* d, const float* aos, uint32_
     if (d->num_points < 2 || d->num_points > HS_MAX_POINTS)
         return fail(c, HS_ERR_INVALID, "hs_groom_create: num_points must be in [2, 256]");
     if (d->num_strands > (1u << 26)) return fail(c, HS_ERR_INVALID, "hs_groom_create: too many strands");
-    // kernels index a component plane with 32-bit offsets: 3*N*Sp < 2^31
-    if ((uint64_t)d->num_points * round_up(d->num_strands, hs::kW) * 3 >= (1ull << 31))
-        return fail(c, HS_ERR_INVALID, "hs_groom_create: more than 715M points in one groom");
+    // kernels address the [3][N][Sp] state with 32-bit byte offsets: 12*N*Sp < 2^32
+    if ((uint64_t)d->num_points * round_up(d->num_strands, hs::kW) * 12 >= (1ull << 32))
+        return fail(c, HS_ERR_INVALID, "hs_groom_create: more than 357M points in one groom");
     if (d->grid_dim > 1024) return fail(c, HS_ERR_INVALID, "hs_groom_create: grid_dim > 1024");
     if (d->grid_dim > 0 && !(d->grid_h > 0.0f && std::isfinite(d->grid_h)))
         return fail(c, HS_ERR_INVALID, "hs_groom_create: grid_h must be finite and > 0");
@@ -406,8 +523,8 @@ int hs_groom_create(hs_ctx* c, const hs_groom_desc* d, const float* aos, uint32_
     }
     HS_CHECK(hipMalloc((void**)&staging, aos_bytes));
 
-    // physical order: Morton order of the roots
-    const std::vector<uint32_t> perm = morton_perm(aos, g.S, g.N);
+    // physical order: k-d leaves of the roots, one per workgroup
+    const std::vector<uint32_t> perm = kd_perm(aos, g.S, g.N);
     std::vector<float> seg(L.Sp, 0.0f), nz;
     // radius = strandLength / (NUM_CURVE_POINTS - 1.0)  (compute.comp:139-140)
     const float r_uniform = d->seg_length ? 0.0f : d->strand_length / ((float)g.N - 1.0f);
@@ -449,78 +566,17 @@ int hs_step(hs_ctx* c, uint32_t id, const hs_time* t, const hs_collider* cols, u
             const hs_step_params* p)
 {
     if (!c) return HS_ERR_INVALID;
-    if (!t || !p) return fail(c, HS_ERR_INVALID, "hs_step: null time or params");
     Groom* g = find(c, id);
     if (!g) return fail(c, HS_ERR_NOT_FOUND, "hs_step: unknown groom");
-    if (ncol > HS_MAX_COLLIDERS) return fail(c, HS_ERR_INVALID, "hs_step: more than 8 colliders");
-    if (ncol > 0 && !cols) return fail(c, HS_ERR_INVALID, "hs_step: null colliders");
-    if (p->num_colliders > ncol) return fail(c, HS_ERR_INVALID, "hs_step: params.num_colliders > colliders given");
-    if (!(t->deltaTime > 0.0f) || !std::isfinite(t->deltaTime))
-        return fail(c, HS_ERR_INVALID, "hs_step: deltaTime must be finite and > 0");
-    if (p->mesh_id != 0) return fail(c, HS_ERR_NOT_FOUND, "hs_step: unknown mesh");
     HS_TRY(c, hipSetDevice(c->device));
-
     hs::StepArgs a;
-    std::memset(&a, 0, sizeof(a));
-    a.pos = g->pos;
-    a.vel = g->vel;
-    a.corr = g->corr;
-    a.seg = g->seg;
-    a.noise = g->noise;
-    a.S = g->S;
-    a.N = g->N;
-    a.Sp = g->L.Sp;
-    a.nwg = g->L.nwg;
-    a.nseg = g->L.nseg;
-    a.gw = g->gw;
-    a.G = g->G;
-    a.parity = g->parity;
-    a.ox = g->origin[0];
-    a.oy = g->origin[1];
-    a.oz = g->origin[2];
-    a.inv_h = g->G ? 1.0f / g->h : 0.0f;
-    a.dt = t->deltaTime;
-    a.inv_dt = 1.0f / a.dt;
-    a.dt2 = a.dt * a.dt;
-    a.D = p->damping;
-    a.k = p->spring_k;
-    a.vmax = p->vmax;
-    a.vmax2 = sqrt_threshold(p->vmax);
-    a.sphere_r = p->sphere_radius;
-    a.fx = p->gravity[0] + p->wind[0];
-    a.fy = p->gravity[1] + p->wind[1];
-    a.fz = p->gravity[2] + p->wind[2];
-    a.ndx = g->noise_dir[0];
-    a.ndy = g->noise_dir[1];
-    a.ndz = g->noise_dir[2];
-    a.ncol = p->num_colliders;
-    a.cols = g->cols;
-    if (a.ncol > 0) {
-        a.sphere_c[0] = cols[0].transform[12];
-        a.sphere_c[1] = cols[0].transform[13];
-        a.sphere_c[2] = cols[0].transform[14];
-    }
-    if (a.ncol > 1) {
-        // ellipsoids are read from device memory; stream order keeps the
-        // previous step's kernel ahead of this copy
-        hs::Collider host[hs::kMaxColliders];
-        for (uint32_t j = 0; j < a.ncol; ++j) {
-            std::memcpy(host[j].m, cols[j].transform, 64);
-            std::memcpy(host[j].m + 16, cols[j].inv, 64);
-            std::memcpy(host[j].m + 32, cols[j].invTrans, 64);
-        }
-        HS_TRY(c, hipMemcpyAsync(g->cols, host, sizeof(hs::Collider) * a.ncol, hipMemcpyHostToDevice, c->stream));
-    }
-    const bool gather = g->pending;
-    if (gather) {
-        a.mu_prev = g->pend_mu;
-        a.omu_prev = g->pend_omu;
-    }
-    const bool splat = p->hair_hair && g->G > 0;
+    bool gather = false, splat = false;
+    int rc = build_args(c, *g, t, cols, ncol, p, "hs_step", &a, &gather, &splat);
+    if (rc != HS_OK) return rc;
 
     hs_ctx::Ev* ev = nullptr;
     if (c->profile) {
-        int rc = next_events(c, &ev);
+        rc = next_events(c, &ev);
         if (rc != HS_OK) return rc;
         ev->reduce = splat;
         HS_TRY(c, hipEventRecord(ev->e[0], c->stream));
@@ -530,13 +586,103 @@ int hs_step(hs_ctx* c, uint32_t id, const hs_time* t, const hs_collider* cols, u
     if (splat) {
         HS_TRY(c, hs::launch_reduce(g->gw, g->G, g->parity, c->stream));
         if (ev) HS_TRY(c, hipEventRecord(ev->e[2], c->stream));
-        g->parity ^= 1u;
-        g->pending = true;
-        g->pend_mu = p->friction;
-        g->pend_omu = 1.0f - p->friction;
-    } else {
-        g->pending = false;
     }
+    advance_state(*g, splat, p->friction);
+    return HS_OK;
+}
+
+int hs_step_batch(hs_ctx* c, const hs_batch_item* items, uint32_t n)
+{
+    if (!c) return HS_ERR_INVALID;
+    if (n == 0) return HS_OK;
+    if (!items) return fail(c, HS_ERR_INVALID, "hs_step_batch: null items");
+    HS_TRY(c, hipSetDevice(c->device));
+    // validate everything before touching any state
+    std::vector<Groom*> gs(n);
+    for (uint32_t k = 0; k < n; ++k) {
+        gs[k] = find(c, items[k].groom_id);
+        if (!gs[k]) return fail(c, HS_ERR_NOT_FOUND, "hs_step_batch: unknown groom");
+        for (uint32_t j = 0; j < k; ++j)
+            if (gs[j] == gs[k]) return fail(c, HS_ERR_INVALID, "hs_step_batch: groom listed twice");
+    }
+    std::vector<hs::StepArgs> args(n);
+    std::vector<uint8_t> variant(n);
+    for (uint32_t k = 0; k < n; ++k) {
+        bool gather = false, splat = false;
+        int rc = build_args(c, *gs[k], &items[k].time, items[k].colliders, items[k].num_colliders, items[k].params,
+                            "hs_step_batch", &args[k], &gather, &splat);
+        if (rc != HS_OK) return rc;
+        variant[k] = (uint8_t)((gather ? 1 : 0) | (splat ? 2 : 0) | (args[k].ncol > 1 ? 4 : 0));
+    }
+    // one launch pair per kernel variant present (usually exactly one)
+    hs_ctx::Ev* ev = nullptr;
+    bool any_splat = false;
+    for (uint32_t k = 0; k < n; ++k) any_splat |= (variant[k] & 2) != 0;
+    if (c->profile) {
+        int rc = next_events(c, &ev);
+        if (rc != HS_OK) return rc;
+        ev->reduce = any_splat;
+        HS_TRY(c, hipEventRecord(ev->e[0], c->stream));
+    }
+    std::vector<std::vector<uint32_t>> groups(8);
+    for (uint32_t k = 0; k < n; ++k) groups[variant[k]].push_back(k);
+    // host staging: args of every group, then the prefix arrays
+    std::vector<hs::StepArgs> hargs;
+    std::vector<uint32_t> hidx;
+    struct Launch {
+        size_t args_off, wg_off, br_off;
+        uint32_t ng, total_wg, total_br;
+        uint8_t v;
+    };
+    std::vector<Launch> launches;
+    for (uint8_t v = 0; v < 8; ++v) {
+        if (groups[v].empty()) continue;
+        Launch L{hargs.size(), 0, 0, (uint32_t)groups[v].size(), 0, 0, v};
+        std::vector<uint32_t> wg{0}, br{0};
+        for (uint32_t k : groups[v]) {
+            hargs.push_back(args[k]);
+            wg.push_back(wg.back() + args[k].nwg);
+            const uint32_t nb = args[k].G ? (args[k].G + hs::kBrick - 1) / hs::kBrick : 0;
+            br.push_back(br.back() + ((v & 2) ? nb * nb * nb : 0));
+        }
+        L.total_wg = wg.back();
+        L.total_br = br.back();
+        L.wg_off = hidx.size();
+        hidx.insert(hidx.end(), wg.begin(), wg.end());
+        L.br_off = hidx.size();
+        hidx.insert(hidx.end(), br.begin(), br.end());
+        launches.push_back(L);
+    }
+    const size_t need_args = hargs.size() * sizeof(hs::StepArgs);
+    const size_t need_idx = hidx.size() * sizeof(uint32_t);
+    if (need_args > c->batch_args_cap || need_idx > c->batch_idx_cap) {
+        HS_TRY(c, hipStreamSynchronize(c->stream));
+        (void)hipFree(c->batch_args);
+        (void)hipFree(c->batch_idx);
+        c->batch_args = nullptr;
+        c->batch_idx = nullptr;
+        c->batch_args_cap = c->batch_idx_cap = 0;
+        HS_TRY(c, hipMalloc((void**)&c->batch_args, std::max<size_t>(need_args, 4096) * 2));
+        HS_TRY(c, hipMalloc((void**)&c->batch_idx, std::max<size_t>(need_idx, 4096) * 2));
+        c->batch_args_cap = std::max<size_t>(need_args, 4096) * 2;
+        c->batch_idx_cap = std::max<size_t>(need_idx, 4096) * 2;
+    }
+    // pageable sources: the copies are staged before returning, and stream
+    // order keeps the previous batch's kernels ahead of them
+    HS_TRY(c, hipMemcpyAsync(c->batch_args, hargs.data(), need_args, hipMemcpyHostToDevice, c->stream));
+    HS_TRY(c, hipMemcpyAsync(c->batch_idx, hidx.data(), need_idx, hipMemcpyHostToDevice, c->stream));
+    for (const Launch& L : launches) {
+        const hs::StepArgs* ad = c->batch_args + L.args_off;
+        HS_TRY(c, hs::launch_step_batch(ad, c->batch_idx + L.wg_off, L.ng, L.total_wg, (L.v & 1) != 0,
+                                        (L.v & 2) != 0, (L.v & 4) != 0, c->stream));
+    }
+    if (ev) HS_TRY(c, hipEventRecord(ev->e[1], c->stream));
+    for (const Launch& L : launches)
+        if (L.v & 2)
+            HS_TRY(c, hs::launch_reduce_batch(c->batch_args + L.args_off, c->batch_idx + L.br_off, L.ng,
+                                              L.total_br, c->stream));
+    if (ev && any_splat) HS_TRY(c, hipEventRecord(ev->e[2], c->stream));
+    for (uint32_t k = 0; k < n; ++k) advance_state(*gs[k], (variant[k] & 2) != 0, items[k].params->friction);
     return HS_OK;
 }
 

@@ -41,7 +41,9 @@ struct StepArgs {
     float ox, oy, oz, inv_h;
     float mu_prev, omu_prev;  // friction of the previous step and 1 - friction
     // step constants
-    float dt, inv_dt, dt2, D, k, vmax, vmax2, sphere_r;  // vmax2: largest x with sqrt(x) <= vmax
+    float dt, inv_dt, dt2, D, k, vmax, sphere_r;
+    float vmax2;      // largest x with sqrt(x) <= vmax
+    float sphere_r2;  // smallest x with sqrt(x) >= sphere_r
     float fx, fy, fz;     // gravity + uniform wind
     float ndx, ndy, ndz;  // per-strand noise direction
     float sphere_c[3];    // collider 0 centre (Collider.transform[3].xyz)
@@ -57,6 +59,12 @@ struct UnpackGrid {
 
 hipError_t launch_step(const StepArgs& a, bool gather, bool splat, hipStream_t st);
 hipError_t launch_reduce(const GridWork& gw, uint32_t G, uint32_t parity, hipStream_t st);
+// Many grooms per launch: per-groom arguments in device memory; wg_start /
+// brick_start are prefix sums (ngroom + 1 entries) of workgroups / bricks.
+hipError_t launch_step_batch(const StepArgs* args_dev, const uint32_t* wg_start_dev, uint32_t ngroom,
+                             uint32_t total_wg, bool gather, bool splat, bool ell, hipStream_t st);
+hipError_t launch_reduce_batch(const StepArgs* args_dev, const uint32_t* brick_start_dev, uint32_t ngroom,
+                               uint32_t total_bricks, hipStream_t st);
 hipError_t launch_pack(const float* aos_dev, const uint32_t* perm, uint32_t S, uint32_t N, uint32_t Sp,
                        float* pos, float* vel, float* corr, hipStream_t st);
 hipError_t launch_unpack(float* aos_dev, const uint32_t* perm, uint32_t S, uint32_t N, uint32_t Sp,

@@ -65,6 +65,25 @@ __device__ __forceinline__ bool rng_full(const Rng& r)
     return r.x1 == r.x0 + 1 && r.y1 == r.y0 + 1 && r.z1 == r.z0 + 1;
 }
 
+// A full 2x2x2 block packed in 31 bits (base voxel, 10 bits per axis, G <= 1024);
+// 0 marks "not a full block".
+__device__ __forceinline__ uint32_t pack_block(const Rng& r)
+{
+    return rng_full(r) ? (0x80000000u | (uint32_t)r.x0 | ((uint32_t)r.y0 << 10) | ((uint32_t)r.z0 << 20)) : 0u;
+}
+
+__device__ __forceinline__ Rng unpack_block(uint32_t p)
+{
+    Rng r;
+    r.x0 = (int)(p & 1023u);
+    r.y0 = (int)((p >> 10) & 1023u);
+    r.z0 = (int)((p >> 20) & 1023u);
+    r.x1 = r.x0 + 1;
+    r.y1 = r.y0 + 1;
+    r.z1 = r.z0 + 1;
+    return r;
+}
+
 // Friction blend (compute.comp:296-297): (1 - mu) * v + mu * v_grid.
 __device__ __forceinline__ f3 friction(f3 v, f3 gv, float mu, float omu) { return add(smul(omu, v), smul(mu, gv)); }
 
@@ -84,6 +103,23 @@ __device__ __forceinline__ int win_index(const Win& w, int a, int b, int c)
     return ((c - w.z0) * w.ey + (b - w.y0)) * w.ex + (a - w.x0);
 }
 
+
+// 32-bit byte-offset accessors of the [3][N][Sp] planes: x at off, y at
+// off + P4, z at off + P8.
+__device__ __forceinline__ f3 ld3(const float* base, uint32_t off, uint32_t P4, uint32_t P8)
+{
+    const char* b = reinterpret_cast<const char*>(base);
+    return mk(*reinterpret_cast<const float*>(b + off), *reinterpret_cast<const float*>(b + (off + P4)),
+              *reinterpret_cast<const float*>(b + (off + P8)));
+}
+
+__device__ __forceinline__ void st3(float* base, uint32_t off, uint32_t P4, uint32_t P8, f3 v)
+{
+    char* b = reinterpret_cast<char*>(base);
+    *reinterpret_cast<float*>(b + off) = v.x;
+    *reinterpret_cast<float*>(b + (off + P4)) = v.y;
+    *reinterpret_cast<float*>(b + (off + P8)) = v.z;
+}
 
 // ---------------------------------------------------------------------------
 // Grid gathers.  Cells are visited x-major, then y, then z as the shader's
@@ -265,8 +301,10 @@ __device__ __forceinline__ f3 force_at(f3 x, f3 fs, const StepArgs& a)
     if (a.ncol > 0) {
         // sphere: compute.comp:160-169 (centre = transform[3].xyz)
         const f3 dv = sub(x, mk(a.sphere_c[0], a.sphere_c[1], a.sphere_c[2]));
-        const float dist = length(dv);
-        if (dist < a.sphere_r) {
+        const float d2 = dot(dv, dv);
+        // d2 < sphere_r2 is exactly sqrt(d2) < sphere_r
+        if (d2 < a.sphere_r2) {
+            const float dist = sqrtf(d2);
             const float d = a.sphere_r - dist;
             const f3 n = muls(dv, 1.0f / dist);
             acc = add(acc, smul(a.k * d, n));
@@ -431,32 +469,34 @@ __device__ __forceinline__ Win chunk_of(const Box& b, const Plan& p, int iy, int
 // ---------------------------------------------------------------------------
 // The step kernel
 // ---------------------------------------------------------------------------
+// XCD-aware order: dispatch places consecutive blockIdx on different XCDs
+// (round robin over 8); give each XCD a contiguous run of logical workgroups
+// (Morton-ordered strand blocks) so neighbouring windows share that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_swizzle(uint32_t block, uint32_t nblocks)
+{
+    const uint32_t per = (nblocks + 7) / 8;
+    return (block % 8) * per + block / 8;
+}
+
+// One workgroup's share of a step: strands [wg*kW, wg*kW + kW) of the groom `a`.
 template <bool GATHER, bool SPLAT, bool ELL>
-__global__ __launch_bounds__(kW) void k_step(StepArgs a)
+__device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
 {
     __shared__ __align__(16) int4 win[kWinCells];
     __shared__ int red[kWaves * 6];
     __shared__ uint32_t bcast[2];
-
-    // XCD-aware order: dispatch places consecutive blockIdx on different XCDs
-    // (round robin over 8); give each XCD a contiguous run of Morton-ordered
-    // strand blocks so neighbouring windows share that XCD's L2.
-    const uint32_t per = (a.nwg + 7) / 8;
-    const uint32_t wg = (blockIdx.x % 8) * per + blockIdx.x / 8;
-    if (wg >= a.nwg) return;
 
     const uint32_t s = wg * kW + threadIdx.x;
     const bool active = s < a.S;
     const int slot = threadIdx.x & (kSlots - 1);
     const uint32_t Sp = a.Sp;
     const uint32_t N = a.N;
-    const uint32_t plane = N * Sp;  // < 2^30 (checked on the host)
-    float* __restrict__ px = a.pos;
-    float* __restrict__ py = a.pos + plane;
-    float* __restrict__ pz = a.pos + 2 * plane;
-    float* __restrict__ vx = a.vel;
-    float* __restrict__ vy = a.vel + plane;
-    float* __restrict__ vz = a.vel + 2 * plane;
+    const uint32_t plane = N * Sp;
+    // byte offsets into the [3][N][Sp] arrays stay below 2^32 (checked on the
+    // host), so every access is an SGPR base plus a 32-bit VGPR offset
+    const uint32_t P4 = plane * 4u, P8 = plane * 8u;
+    float* __restrict__ posb = a.pos;
+    float* __restrict__ velb = a.vel;
     const f3 origin = mk(a.ox, a.oy, a.oz);
     const int G = (int)a.G;
 
@@ -464,7 +504,7 @@ __global__ __launch_bounds__(kW) void k_step(StepArgs a)
     float r = 0.0f;
     f3 fs = mk(a.fx, a.fy, a.fz);
     if (active) {
-        par = mk(px[s], py[s], pz[s]);
+        par = ld3(posb, s * 4u, P4, P8);
         r = a.seg[s];
         if (a.noise) fs = add(fs, smul(a.noise[s], mk(a.ndx, a.ndy, a.ndz)));
     }
@@ -475,16 +515,16 @@ __global__ __launch_bounds__(kW) void k_step(StepArgs a)
         const uint32_t f = 1 + sg * kL;
         const uint32_t cnt = min(kL, N - f);
         const bool last_seg = f + cnt == N;
-        const uint32_t o0 = f * Sp + s;
+        const uint32_t b0 = (f * Sp + s) * 4u;  // byte offset of point f
+        const uint32_t bS = Sp * 4u;             // byte stride between points
         f3 X[kL], U[kL];
 #pragma unroll
         for (uint32_t t = 0; t < kL; ++t) {
             X[t] = mk(0.0f, 0.0f, 0.0f);
             U[t] = mk(0.0f, 0.0f, 0.0f);
             if (active && t < cnt) {
-                const uint32_t o = o0 + t * Sp;
-                X[t] = mk(px[o], py[o], pz[o]);
-                U[t] = mk(vx[o], vy[o], vz[o]);
+                X[t] = ld3(posb, b0 + t * bS, P4, P8);
+                U[t] = ld3(velb, b0 + t * bS, P4, P8);
             }
         }
 
@@ -506,17 +546,18 @@ __global__ __launch_bounds__(kW) void k_step(StepArgs a)
                 const int cells = gwin.ex * gwin.ey * gwin.ez;
                 float4* fw = reinterpret_cast<float4*>(win);
                 // loads for up to 8 cells per thread in flight before the LDS stores
+                const uint32_t GG = (uint32_t)G * (uint32_t)G;
                 for (int c0 = 0; c0 < cells; c0 += 8 * kW) {
+                    // unconditional (clamped) loads: all 8 in flight before the stores
                     float4 tmp[8];
 #pragma unroll
                     for (int k = 0; k < 8; ++k) {
-                        const int c = c0 + k * kW + threadIdx.x;
-                        if (c < cells) {
-                            const int ax = c % gwin.ex;
-                            const int rest = c / gwin.ex;
-                            tmp[k] = a.gw.grid_f[(size_t)(gwin.x0 + ax) + (size_t)(gwin.y0 + rest % gwin.ey) * G +
-                                                 (size_t)(gwin.z0 + rest / gwin.ey) * G * G];
-                        }
+                        const uint32_t c = (uint32_t)min(c0 + k * (int)kW + (int)threadIdx.x, cells - 1);
+                        const uint32_t ax = c % (uint32_t)gwin.ex;
+                        const uint32_t rest = c / (uint32_t)gwin.ex;
+                        const uint32_t gi = (uint32_t)gwin.x0 + ax + ((uint32_t)gwin.y0 + rest % (uint32_t)gwin.ey) * (uint32_t)G +
+                                            ((uint32_t)gwin.z0 + rest / (uint32_t)gwin.ey) * GG;
+                        tmp[k] = a.gw.grid_f[gi];
                     }
 #pragma unroll
                     for (int k = 0; k < 8; ++k) {
@@ -556,11 +597,14 @@ __global__ __launch_bounds__(kW) void k_step(StepArgs a)
 
         // ---------------- F: FTL, velocity, DFTL (the root-to-tip chain) ----------------
         Box nbox = box_empty();  // voxels of this segment's new positions (next step's gather)
+        uint32_t RP[kL];         // packed voxel block of each new position (0: recompute)
+#pragma unroll
+        for (uint32_t t = 0; t < kL; ++t) RP[t] = 0;
         if (active) {
 #pragma unroll
             for (uint32_t t = 0; t < kL; ++t) {
                 if (t >= cnt) continue;
-                const uint32_t o = o0 + t * Sp;
+                const uint32_t bo = b0 + t * bS;
                 const f3 x = X[t];
                 const f3 pred = U[t];
                 // follow the leader (compute.comp:191-192)
@@ -570,39 +614,28 @@ __global__ __launch_bounds__(kW) void k_step(StepArgs a)
                 f3 vel = smul(a.D, muls(sub(pn, x), a.inv_dt));
                 if (dot(vel, vel) > a.vmax2) vel = muls(normalize(vel), a.vmax);
                 const f3 corr = smul(a.D, sub(pn, pred));
-                px[o] = pn.x;
-                py[o] = pn.y;
-                pz[o] = pn.z;
-                if (a.corr) {
-                    a.corr[o] = corr.x;
-                    a.corr[plane + o] = corr.y;
-                    a.corr[2 * plane + o] = corr.z;
-                }
+                st3(posb, bo, P4, P8, pn);
+                if (a.corr) st3(a.corr, bo, P4, P8, corr);
                 // DFTL for point f+t-1 (compute.comp:213-215)
                 const f3 dc = muls(corr, a.inv_dt);
                 if (t == 0) {
                     if (f >= 2) {
                         cv = sub(cv, dc);
-                        vx[o - Sp] = cv.x;
-                        vy[o - Sp] = cv.y;
-                        vz[o - Sp] = cv.z;
+                        st3(velb, bo - bS, P4, P8, cv);
                     }
                 } else {
                     U[t - 1] = sub(U[t - 1], dc);
-                    vx[o - Sp] = U[t - 1].x;
-                    vy[o - Sp] = U[t - 1].y;
-                    vz[o - Sp] = U[t - 1].z;
+                    st3(velb, bo - bS, P4, P8, U[t - 1]);
                 }
                 X[t] = pn;
                 U[t] = vel;
                 par = pn;
-                if (SPLAT) box_add(nbox, point_range(grid_coord(pn, origin, a.inv_h), G));
-                if (t + 1 == cnt && last_seg) {
-                    // tip: no correction (compute.comp:213)
-                    vx[o] = vel.x;
-                    vy[o] = vel.y;
-                    vz[o] = vel.z;
+                if (SPLAT) {
+                    const Rng rn = point_range(grid_coord(pn, origin, a.inv_h), G);
+                    box_add(nbox, rn);
+                    RP[t] = pack_block(rn);
                 }
+                if (t + 1 == cnt && last_seg) st3(velb, bo, P4, P8, vel);  // tip: no correction (compute.comp:213)
             }
         }
 
@@ -615,7 +648,7 @@ __global__ __launch_bounds__(kW) void k_step(StepArgs a)
 #pragma unroll
                 for (uint32_t t = 0; t < kL; ++t)
                     if (t + 1 < cnt || (t + 1 == cnt && last_seg))
-                        box_add(sbox, point_range(grid_coord(X[t], origin, a.inv_h), G));
+                        box_add(sbox, RP[t] ? unpack_block(RP[t]) : point_range(grid_coord(X[t], origin, a.inv_h), G));
             }
             const Box gbn = block_box(nbox, red);
             if (threadIdx.x == 0) {
@@ -654,6 +687,26 @@ __global__ __launch_bounds__(kW) void k_step(StepArgs a)
                     for (int iy = 0; iy < pl.ny; ++iy) {
                         const Win cw = chunk_of(box, pl, iy, iz);
                         const int cells = cw.ex * cw.ey * cw.ez;
+                        if (pl.ny * pl.nz > 1) {
+                            // several chunks: skip the ones no remaining point falls in
+                            bool mine = false;
+                            if (active) {
+                                if (f >= 2 && !((done >> kL) & 1u)) {
+                                    const Rng rg = point_range(grid_coord(cp, origin, a.inv_h), G);
+                                    mine |= !rng_empty(rg) && in_win(rg, cw);
+                                }
+#pragma unroll
+                                for (uint32_t t = 0; t < kL; ++t) {
+                                    if (!(t + 1 < cnt || (t + 1 == cnt && last_seg)) || ((done >> t) & 1u)) continue;
+                                    f3 xp = X[t];
+                                    uint32_t rp = RP[t];
+                                    asm volatile("" : "+v"(xp.x), "+v"(xp.y), "+v"(xp.z), "+v"(rp));
+                                    const Rng rg = rp ? unpack_block(rp) : point_range(grid_coord(xp, origin, a.inv_h), G);
+                                    mine |= !rng_empty(rg) && in_win(rg, cw);
+                                }
+                            }
+                            if (!__syncthreads_or(mine)) continue;
+                        }
                         for (int c = threadIdx.x; c < cells * kSlots; c += kW) win[c] = make_int4(0, 0, 0, 0);
                         __syncthreads();
                         if (active) {
@@ -675,9 +728,10 @@ __global__ __launch_bounds__(kW) void k_step(StepArgs a)
                                 // opaque copy: keeps the per-point voxel math inside the
                                 // chunk loop instead of hoisting 16 points' worth of it
                                 f3 xp = X[t];
-                                asm volatile("" : "+v"(xp.x), "+v"(xp.y), "+v"(xp.z));
+                                uint32_t rp = RP[t];
+                                asm volatile("" : "+v"(xp.x), "+v"(xp.y), "+v"(xp.z), "+v"(rp));
                                 const f3 q = grid_coord(xp, origin, a.inv_h);
-                                const Rng rg = point_range(q, G);
+                                const Rng rg = rp ? unpack_block(rp) : point_range(q, G);
                                 if (rng_empty(rg) || !in_win(rg, cw)) continue;
                                 if (rng_full(rg))
                                     splat_block(win, cw, rg, q, U[t], slot);
@@ -750,13 +804,50 @@ __global__ __launch_bounds__(kW) void k_step(StepArgs a)
     }
 }
 
+template <bool GATHER, bool SPLAT, bool ELL>
+__global__ __launch_bounds__(kW) void k_step(StepArgs a)
+{
+    const uint32_t wg = xcd_swizzle(blockIdx.x, a.nwg);
+    if (wg >= a.nwg) return;
+    step_body<GATHER, SPLAT, ELL>(a, wg);
+}
+
+// Many grooms in one launch: wg_start[g] is the first global workgroup of
+// groom g (prefix sum of their workgroup counts, ngroom+1 entries).
+__device__ __forceinline__ uint32_t find_groom(const uint32_t* __restrict__ start, uint32_t n, uint32_t x)
+{
+    uint32_t lo = 0, hi = n;  // start[lo] <= x < start[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (start[mid] <= x)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+template <bool GATHER, bool SPLAT, bool ELL>
+__global__ __launch_bounds__(kW) void k_step_batch(const StepArgs* __restrict__ args,
+                                                   const uint32_t* __restrict__ wg_start, uint32_t ngroom)
+{
+    const uint32_t total = wg_start[ngroom];
+    const uint32_t gwg = xcd_swizzle(blockIdx.x, total);
+    if (gwg >= total) return;
+    const uint32_t g = find_groom(wg_start, ngroom, gwg);
+    // by-value copy: the arguments then live in SGPRs like a kernel argument
+    const StepArgs a = args[g];
+    step_body<GATHER, SPLAT, ELL>(a, gwg - wg_start[g]);
+}
+
 // ---------------------------------------------------------------------------
 // Reduce: partial boxes -> grid (GridCell ints + float cells); clears the grid.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kReduceThreads = 256;
 constexpr uint32_t kListCap = 4096;
 
-__global__ __launch_bounds__(kReduceThreads) void k_reduce(GridWork gw, uint32_t G, uint32_t parity)
+__device__ __forceinline__ void reduce_body(const GridWork& gw, uint32_t G, uint32_t parity, uint32_t brick,
+                                            bool reset_counters)
 {
     constexpr int kCells = kBrick * kBrick * kBrick;
     constexpr int kRWaves = kReduceThreads / 64;
@@ -764,7 +855,7 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce(GridWork gw, uint32_t
     __shared__ uint32_t nlist;
     __shared__ uint32_t acc[kCells][4];
     const uint32_t nb = (G + kBrick - 1) / kBrick;
-    const uint32_t bx = blockIdx.x % nb, by = (blockIdx.x / nb) % nb, bz = blockIdx.x / (nb * nb);
+    const uint32_t bx = brick % nb, by = (brick / nb) % nb, bz = brick / (nb * nb);
     const int x0 = bx * kBrick, y0 = by * kBrick, z0 = bz * kBrick;
     const int x1 = min(x0 + (int)kBrick, (int)G) - 1, y1 = min(y0 + (int)kBrick, (int)G) - 1,
               z1 = min(z0 + (int)kBrick, (int)G) - 1;
@@ -843,12 +934,29 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce(GridWork gw, uint32_t
         gw.grid_i[gi] = v;
         gw.grid_f[gi] = make_float4((float)v.x, (float)v.y, (float)v.z, v.w > 0 ? 1.0f / (float)v.w : 0.0f);
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (reset_counters && threadIdx.x == 0) {
         uint32_t* nxt = gw.counters + (parity ^ 1u) * 4;
         nxt[0] = 0;
         nxt[1] = 0;
         nxt[2] = 0;
     }
+}
+
+__global__ __launch_bounds__(kReduceThreads) void k_reduce(GridWork gw, uint32_t G, uint32_t parity)
+{
+    reduce_body(gw, G, parity, blockIdx.x, blockIdx.x == 0);
+}
+
+// Many grooms: brick_start[g] is the first global brick of groom g.
+__global__ __launch_bounds__(kReduceThreads) void k_reduce_batch(const StepArgs* __restrict__ args,
+                                                                 const uint32_t* __restrict__ brick_start,
+                                                                 uint32_t ngroom)
+{
+    const uint32_t total = brick_start[ngroom];
+    if (blockIdx.x >= total) return;
+    const uint32_t g = find_groom(brick_start, ngroom, blockIdx.x);
+    const uint32_t b = blockIdx.x - brick_start[g];
+    reduce_body(args[g].gw, args[g].G, args[g].parity, b, b == 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -940,6 +1048,42 @@ hipError_t launch_step(const StepArgs& a, bool gather, bool splat, hipStream_t s
         else
             launch_step_e<false, false>(a, ell, blocks, st);
     }
+    return hipGetLastError();
+}
+
+template <bool G, bool S>
+static void launch_batch_e(const StepArgs* args, const uint32_t* wg_start, uint32_t ngroom, bool ell,
+                           uint32_t blocks, hipStream_t st)
+{
+    if (ell)
+        hipLaunchKernelGGL((k_step_batch<G, S, true>), dim3(blocks), dim3(kW), 0, st, args, wg_start, ngroom);
+    else
+        hipLaunchKernelGGL((k_step_batch<G, S, false>), dim3(blocks), dim3(kW), 0, st, args, wg_start, ngroom);
+}
+
+hipError_t launch_step_batch(const StepArgs* args_dev, const uint32_t* wg_start_dev, uint32_t ngroom,
+                             uint32_t total_wg, bool gather, bool splat, bool ell, hipStream_t st)
+{
+    const uint32_t blocks = ((total_wg + 7) / 8) * 8;
+    if (gather) {
+        if (splat)
+            launch_batch_e<true, true>(args_dev, wg_start_dev, ngroom, ell, blocks, st);
+        else
+            launch_batch_e<true, false>(args_dev, wg_start_dev, ngroom, ell, blocks, st);
+    } else {
+        if (splat)
+            launch_batch_e<false, true>(args_dev, wg_start_dev, ngroom, ell, blocks, st);
+        else
+            launch_batch_e<false, false>(args_dev, wg_start_dev, ngroom, ell, blocks, st);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_batch(const StepArgs* args_dev, const uint32_t* brick_start_dev, uint32_t ngroom,
+                               uint32_t total_bricks, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_reduce_batch, dim3(total_bricks), dim3(kReduceThreads), 0, st, args_dev, brick_start_dev,
+                       ngroom);
     return hipGetLastError();
 }
 

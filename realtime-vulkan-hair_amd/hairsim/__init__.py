@@ -99,6 +99,16 @@ class HsStepParams(C.Structure):
     ]
 
 
+class HsBatchItem(C.Structure):
+    _fields_ = [
+        ("groom_id", C.c_uint32),
+        ("time", HsTime),
+        ("colliders", C.POINTER(HsCollider)),
+        ("num_colliders", C.c_uint32),
+        ("params", C.POINTER(HsStepParams)),
+    ]
+
+
 # Every symbol include/hairsim.h declares, with its ctypes signature.
 _P = C.c_void_p
 _SIGS = {
@@ -115,6 +125,7 @@ _SIGS = {
     "hs_groom_bytes": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
     "hs_step": (C.c_int, [_P, C.c_uint32, C.POINTER(HsTime), C.POINTER(HsCollider), C.c_uint32,
                           C.POINTER(HsStepParams)]),
+    "hs_step_batch": (C.c_int, [_P, C.POINTER(HsBatchItem), C.c_uint32]),
     "hs_read_strands": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_float), C.POINTER(HsDrawIndirect)]),
     "hs_write_strands": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_float)]),
     "hs_read_grid": (C.c_int, [_P, C.c_uint32, C.POINTER(HsGridCell)]),
@@ -244,6 +255,22 @@ class Context:
         ms = C.c_double()
         self._check(lib().hs_profile_read(self._h, C.byref(n), C.byref(ms)))
         return n.value, ms.value
+
+    def step_batch(self, entries):
+        """entries: list of (groom, dt, total_time, colliders, params)."""
+        keep = []
+        items = (HsBatchItem * max(1, len(entries)))()
+        for k, (groom, dt, tt, cols, params) in enumerate(entries):
+            cl = [c if isinstance(c, HsCollider) else HsCollider.from_buffer_copy(
+                np.ascontiguousarray(c, dtype=np.float32).tobytes()) for c in (cols or [])]
+            arr = (HsCollider * max(1, len(cl)))(*cl)
+            keep.append((arr, params))
+            items[k].groom_id = groom.id
+            items[k].time = HsTime(dt, tt)
+            items[k].colliders = C.cast(arr, C.POINTER(HsCollider))
+            items[k].num_colliders = len(cl)
+            items[k].params = C.pointer(params)
+        self._check(lib().hs_step_batch(self._h, items, len(entries)))
 
     def profile_breakdown(self):
         a = C.c_double()
