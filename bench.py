@@ -11,11 +11,17 @@ the timed region starts.
 
 Prints ONE JSON line.  `value` is point-updates/s (S*N per step) over the
 timed steps (wall clock between two stream synchronisations).  `roofline`
-prices the step kernel's algorithmic bytes (48 B per non-root point + 16 B per
-strand + 32 B per voxel) over its average duration, measured with HIP events
-the library records on the stream the kernel runs on.  `cpu_baseline` times
-the oracle's FP32 restatement of the reference shader on the host cores on a
-bounded sample of the same workload.
+prices the dominant kernel, the strand step kernel, by its algorithmic bytes
+(48 B per non-root point + 16 B per strand) over its average duration,
+measured with HIP events the library records on the stream the kernel runs
+on; the voxel reduce kernel's share (32 B per voxel) and the whole step's
+figure are given beside it.  `cpu_baseline` times the oracle's FP32
+restatement of the reference shader on the host cores on a bounded sample of
+the same workload.
+
+--workload batched-256: 256 grooms of uneven shapes stepped together with one
+captured graph launch per step, reported beside the same grooms stepped with
+one launch pair per groom (`per_groom_launch`).
 """
 from __future__ import annotations
 
@@ -68,23 +74,30 @@ def cpu_baseline(wl, budget_s=12.0):
                       f"(FP32 C restatement of compute.comp, OpenMP)"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=240)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="plain-100k", choices=["plain-100k", "large-1m", "plain-1k"])
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
-              "the strand step has no multi-GPU path here", file=sys.stderr)
-        sys.exit(2)
+def copy_probe(mib, reps=5):
+    """A plain elementwise kernel of known traffic (read n bytes, write n bytes):
+    the calibration that scales the HBM byte counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        torch.add(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "reps": reps}),
+          flush=True)
 
-    import hairsim
-    from hairsim import scenes
 
-    wl = {"plain-100k": scenes.plain_100k, "large-1m": scenes.large_1m, "plain-1k": scenes.plain_1k}[args.workload]()
+def load_traffic(name):
+    prof = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
+    if not os.path.exists(prof):
+        return None
+    with open(prof) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def bench_single(args, hairsim, scenes, wl):
     g = wl.grooms[0]
     ctx = hairsim.Context(0)
     groom = ctx.groom(g.strands, g.N, seg_length=g.seg, wind_noise=g.noise, noise_dir=g.noise_dir,
@@ -108,15 +121,18 @@ def main():
     ctx.sync()
     el = time.perf_counter() - t0
 
-    # kernel durations: same number of steps again with event timing on
+    # kernel durations: the same number of steps again with HIP events on the
+    # library's stream around each kernel
     ctx.profile_enable(True)
     for _ in range(args.steps):
         groom.step(wl.dt, tms[k], cols[k], prm[k])
         k += 1
     ctx.sync()
-    n_launch, kern_ms = ctx.profile_read()
+    n_launch, _ = ctx.profile_read()
+    step_ms, reduce_ms = ctx.profile_breakdown()
     ctx.profile_enable(False)
     out, _ = groom.read()
+    near = groom.grid_near_limit() if g.grid_dim and wl.hair_hair else 0
     if not np.isfinite(out).all():
         print("bench.py: non-finite state after the run", file=sys.stderr)
         sys.exit(1)
@@ -125,14 +141,13 @@ def main():
 
     pts = g.S * g.N
     value = pts * args.steps / el
-    avg_kernel_s = kern_ms / 1e3 / max(1, n_launch)
-    bstep = scenes.algorithmic_bytes(g.S, g.N, g.grid_dim, wl.hair_hair)
-    achieved = bstep / avg_kernel_s / 1e9
-    traffic = None
-    prof = os.path.join(ROOT, "profiles", f"traffic_{wl.name}.json")
-    if os.path.exists(prof):
-        with open(prof) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    n = max(1, n_launch)
+    step_us = step_ms / n * 1e3
+    reduce_us = reduce_ms / n * 1e3
+    b_step_kernel = scenes.strand_bytes(g.S, g.N)
+    b_grid = scenes.algorithmic_bytes(g.S, g.N, g.grid_dim, wl.hair_hair) - b_step_kernel
+    achieved = b_step_kernel / (step_us * 1e-6) / 1e9
+    whole = (b_step_kernel + b_grid) / ((step_us + reduce_us) * 1e-6) / 1e9
     res = {
         "metric": "strand-point updates/sec",
         "value": value,
@@ -150,12 +165,149 @@ def main():
                                f"wind + noise, orbiting sphere, friction {wl.friction}",
                    "strands": g.S, "points": g.N, "grid_dim": g.grid_dim, "hair_hair": wl.hair_hair},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
-                     "kernel_avg_us": avg_kernel_s * 1e6, "algorithmic_bytes_per_launch": bstep},
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": load_traffic(wl.name),
+                     "kernel": "k_step", "kernel_avg_us": step_us, "algorithmic_bytes_per_launch": b_step_kernel,
+                     "reduce_kernel_avg_us": reduce_us, "reduce_algorithmic_bytes": b_grid,
+                     "whole_step_achieved": whole, "whole_step_frac": whole / PEAK_HBM_GBS},
+        "grid_cells_near_int32_limit": near,
     }
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(wl)
     print(json.dumps(res), flush=True)
+
+
+def bench_batched(args, hairsim, scenes, wl):
+    """Many grooms per step: hs_step_batch with captured graphs, against the
+    same grooms stepped with one hs_step launch pair per groom."""
+    ctx = hairsim.Context(0)
+    grooms = [ctx.groom(g.strands, g.N, seg_length=g.seg, wind_noise=g.noise, noise_dir=g.noise_dir,
+                        grid_dim=g.grid_dim, grid_origin=g.grid_origin, grid_h=g.grid_h) for g in wl.grooms]
+    n = len(grooms)
+    ctx.set_batch_graphs(not args.no_graphs)
+    b = ctx.batch(grooms, max_colliders=1)
+    b.params["damping"] = wl.damping
+    b.params["sphere_radius"] = wl.sphere_radius
+    b.params["friction"] = wl.friction
+    b.params["hair_hair"] = 1 if wl.hair_hair else 0
+    b.params["wind"] = np.stack([wl.wind(0, gi) for gi in range(n)])
+    b.set_colliders(np.stack([scenes.sphere_collider(wl.sphere(0, gi)) for gi in range(n)])[:, None])
+
+    def batch_step(k):
+        b.set_time(wl.dt, float(wl.time(k % wl.steps)))
+        b.step()
+
+    k = 0
+    for _ in range(args.warmup):
+        batch_step(k)
+        k += 1
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch_step(k)
+        k += 1
+    ctx.sync()
+    el = time.perf_counter() - t0
+    # per-kernel durations from an eager profiled pass
+    ctx.set_batch_graphs(False)
+    ctx.profile_enable(True)
+    prof_steps = min(args.steps, 20)
+    for _ in range(prof_steps):
+        batch_step(k)
+        k += 1
+    ctx.sync()
+    step_ms, reduce_ms = ctx.profile_breakdown()
+    ctx.profile_enable(False)
+
+    # the plain form: one hs_step per groom per step (inputs prebuilt)
+    lib = hairsim.lib()
+    per = []
+    for gi, gr in enumerate(grooms):
+        t = hairsim.HsTime(wl.dt, 0.0)
+        cl = (hairsim.HsCollider * 1)(hairsim.HsCollider.from_buffer_copy(
+            scenes.sphere_collider(wl.sphere(0, gi)).tobytes()))
+        p = hairsim.step_params(wind=wl.wind(0, gi), damping=wl.damping, sphere_radius=wl.sphere_radius,
+                                friction=wl.friction, num_colliders=1, hair_hair=wl.hair_hair)
+        per.append((gr.id, t, cl, p))
+    import ctypes as C
+    plain_steps = max(3, min(args.steps // 4, 30))
+    for gid, t, cl, p in per:
+        ctx._check(lib.hs_step(ctx._h, gid, C.byref(t), cl, 1, C.byref(p)))
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(plain_steps):
+        for gid, t, cl, p in per:
+            ctx._check(lib.hs_step(ctx._h, gid, C.byref(t), cl, 1, C.byref(p)))
+    ctx.sync()
+    el_plain = time.perf_counter() - t0
+    for gr in grooms:
+        gr.destroy()
+    ctx.close()
+
+    pts = sum(g.S * g.N for g in wl.grooms)
+    value = pts * args.steps / el
+    plain = pts * plain_steps / el_plain
+    b_step_kernel = sum(scenes.strand_bytes(g.S, g.N) for g in wl.grooms)
+    b_all = sum(scenes.algorithmic_bytes(g.S, g.N, g.grid_dim, wl.hair_hair) for g in wl.grooms)
+    step_us = step_ms / prof_steps * 1e3
+    reduce_us = reduce_ms / prof_steps * 1e3
+    achieved = b_step_kernel / (step_us * 1e-6) / 1e9
+    whole = b_all / ((step_us + reduce_us) * 1e-6) / 1e9
+    res = {
+        "metric": "strand-point updates/sec",
+        "value": value,
+        "unit": "point-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{wl.name}: {n} grooms, {sum(g.S for g in wl.grooms)} strands, "
+                               f"{pts} points, grids {wl.grooms[0].grid_dim}^3, one batched step per frame"
+                               f"{'' if args.no_graphs else ' (captured graph)'}",
+                   "grooms": n, "points": pts},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": load_traffic(wl.name),
+                     "kernel": "k_step_batch", "kernel_avg_us": step_us,
+                     "algorithmic_bytes_per_launch": b_step_kernel, "reduce_kernel_avg_us": reduce_us,
+                     "whole_step_achieved": whole, "whole_step_frac": whole / PEAK_HBM_GBS},
+        "per_groom_launch": {"value": plain, "steps": plain_steps, "ms_per_step": el_plain / plain_steps * 1e3,
+                             "speedup_of_batched": value / plain},
+    }
+    print(json.dumps(res), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=240)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="plain-100k",
+                    choices=["plain-100k", "large-1m", "plain-1k", "batched-256"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true", help="batched: eager launches instead of graphs")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="run only a plain elementwise kernel over MIB MiB (byte-counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "the strand step has no multi-GPU path here", file=sys.stderr)
+        sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
+
+    import hairsim
+    from hairsim import scenes
+
+    if args.workload == "batched-256":
+        bench_batched(args, hairsim, scenes, scenes.batched())
+        return
+    wl = {"plain-100k": scenes.plain_100k, "large-1m": scenes.large_1m, "plain-1k": scenes.plain_1k}[args.workload]()
+    bench_single(args, hairsim, scenes, wl)
 
 
 if __name__ == "__main__":

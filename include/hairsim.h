@@ -204,6 +204,16 @@ typedef struct hs_batch_item {
 
 int hs_step_batch(hs_ctx* ctx, const hs_batch_item* items, uint32_t num_items);
 
+/* When on, hs_step_batch replays a captured HIP graph (argument copy, step
+ * kernels, reduce kernels) for each batch shape it has seen: one graph launch
+ * per step instead of one API call per kernel.  A shape is the list of groom
+ * ids with their kernel variants; up to 16 shapes are kept.  Results are the
+ * same bit for bit either way.  No reference counterpart (the reference
+ * re-submits one pre-recorded command buffer, Renderer.cpp:2022-2077, 2317). */
+int hs_set_batch_graphs(hs_ctx* ctx, int on);
+/* hs_step_batch calls so far and the graphs captured for them. */
+int hs_batch_stats(const hs_ctx* ctx, uint64_t* batch_calls, uint64_t* graph_captures);
+
 /* Read the groom back in the reference Strand layout (48*N B per strand) with
  * the velocities after this step's grid friction, plus the indirect draw
  * record {S, 1, 0, 0} the shader's count at compute.comp:302 produces.
@@ -216,6 +226,11 @@ int hs_write_strands(hs_ctx* ctx, uint32_t groom_id, const float* strands_aos);
 /* Read the voxel grid the last step splatted (compute.comp:217-252), G^3
  * cells in GridCell layout, index a + b*G + c*G*G.  Blocks. */
 int hs_read_grid(hs_ctx* ctx, uint32_t groom_id, hs_grid_cell* cells);
+/* Number of (cell, step) pairs since the groom was created whose GridCell
+ * held a component of magnitude >= 2^30, i.e. within 2x of the int32 limit
+ * of the shader's fixed-point sums (compute.comp:235-251, SCALE 1e6): the
+ * format wraps past it, and it is kept as is.  Blocks. */
+int hs_grid_near_limit(hs_ctx* ctx, uint32_t groom_id, uint64_t* cells);
 
 /* ------------------------------------------------------------------------ */
 /* Kernel timing (no reference counterpart: the reference has no timers)     */
@@ -241,6 +256,80 @@ void hs_collider_from_trs(hs_collider* out, const float translation[3],
 /* Scene::translateSphere: transform = transform * translate(t); inv and
  * invTrans recomputed. */
 void hs_collider_translate(hs_collider* c, const float translation[3]);
+
+/* ------------------------------------------------------------------------ */
+/* Served simulation jobs (no reference counterpart: the reference steps one  */
+/* groom per frame, main.cpp:257-285).  A server owns a context, an HBM arena */
+/* of `hbm_budget` bytes and three threads: admission (uploads jobs in FIFO   */
+/* order while the budget and the per-step point limit allow), the step loop  */
+/* (one hs_step_batch over every admitted job per launch, so jobs that arrive */
+/* join the next launch) and completion (reads finished jobs back).           */
+/* ------------------------------------------------------------------------ */
+
+/* Wind and collider-0 centre at time t (seconds since the job's start),
+ * linear between keyframes, held constant before the first and after the
+ * last one. */
+typedef struct hs_keyframe {
+    float t;
+    float wind[3];
+    float sphere[3];
+} hs_keyframe;
+
+typedef struct hs_job {
+    hs_groom_desc groom;          /* shape, rest lengths, noise, grid           */
+    const float* strands_aos;     /* initial state, reference Strand layout     */
+    uint32_t steps;               /* steps to run                               */
+    float dt;                     /* step size; step k sees totalTime (k+1)*dt  */
+    const hs_keyframe* keys;      /* wind/sphere trajectory (>= 1 key)          */
+    uint32_t num_keys;
+    hs_step_params params;        /* wind and collider 0 come from the keys;    */
+                                  /* num_colliders is 0 or 1 (the sphere)        */
+    float* result_aos;            /* receives the final state, 48*S*N bytes     */
+} hs_job;
+/* strands_aos, keys and result_aos are the caller's and must stay valid until
+ * the job is reported done (hs_server_wait / hs_server_drain). */
+
+typedef struct hs_server_config {
+    int device;
+    uint64_t hbm_budget;          /* device bytes of resident job state         */
+    uint64_t max_points_per_step; /* points in one batched launch (latency cap) */
+    int use_graphs;               /* replay captured graphs for repeated shapes */
+} hs_server_config;
+
+typedef struct hs_server_stats {
+    uint64_t submitted, admitted, completed, cancelled;
+    uint64_t launches;            /* batched step launches                      */
+    uint64_t job_steps;           /* sum over launches of the jobs stepped      */
+    uint64_t point_updates;       /* sum over launches of S*N stepped           */
+    double p50_step_ms, p99_step_ms, max_step_ms;   /* per job-step, launch to  */
+                                                    /* completion on the device */
+    double p50_wait_ms, p99_wait_ms;                /* submit -> first step     */
+    double busy_s;                /* time with at least one job admitted        */
+    uint64_t peak_resident_bytes;
+    uint32_t peak_jobs_per_launch;
+} hs_server_stats;
+
+typedef struct hs_server hs_server;
+
+int hs_server_create(const hs_server_config* cfg, hs_server** out);
+/* Queues a job; returns at once.  HS_ERR_BUDGET if the job alone exceeds
+ * the budget or the per-step point limit. */
+int hs_server_submit(hs_server* srv, const hs_job* job, uint64_t* job_id);
+/* Waits until the job is done (0), or timeout_s passes (HS_ERR_STATE);
+ * a negative timeout waits without limit.  HS_ERR_NOT_FOUND for an unknown
+ * or cancelled job. */
+int hs_server_wait(hs_server* srv, uint64_t job_id, double timeout_s);
+/* Drops queued jobs that are not admitted yet (their results stay unwritten). */
+int hs_server_cancel_pending(hs_server* srv, uint64_t* cancelled);
+/* Waits until every admitted job is done. */
+int hs_server_drain(hs_server* srv);
+int hs_server_get_stats(hs_server* srv, hs_server_stats* out);
+/* Status and message of the first failure on a server thread (0 if none). */
+int hs_server_error(hs_server* srv, const char** message);
+/* Cancels what is queued, finishes what is admitted, frees everything. */
+void hs_server_destroy(hs_server* srv);
+/* The keyframe interpolation the server applies (for stepping a job alone). */
+void hs_keyframes_eval(const hs_keyframe* keys, uint32_t num_keys, float t, float wind[3], float sphere[3]);
 
 #ifdef __cplusplus
 }

@@ -20,92 +20,102 @@
 #include "../../include/hairsim.h"
 #include "hs_kernels.h"
 
-namespace {
+#include "hs_internal.h"
 
-uint64_t round_up(uint64_t x, uint64_t m) { return (x + m - 1) / m * m; }
-uint64_t grid_cells(uint32_t G) { return (uint64_t)G * G * G; }
+using namespace hsi;
 
-// Shape-derived sizes of one groom's device state.
-struct Layout {
-    uint32_t Sp, nwg, nseg;
-    uint64_t pool_cap, desc_cap;
-    uint64_t state_bytes, grid_bytes;
-};
+namespace hsi {
 
-Layout layout_of(uint32_t S, uint32_t N, uint32_t G, uint32_t flags, bool noise)
+Layout layout_of(uint32_t S, uint32_t N, uint32_t G)
 {
     Layout l;
     l.Sp = (uint32_t)round_up(S, hs::kW);
     l.nwg = l.Sp / hs::kW;
     l.nseg = (N - 1 + hs::kL - 1) / hs::kL;
-    const uint64_t planes = 3ull * N * l.Sp * 4;
-    l.state_bytes = 2 * planes + (uint64_t)l.Sp * 4 /*seg*/ + (uint64_t)l.Sp * 4 /*perm*/ +
-                    sizeof(hs::Collider) * hs::kMaxColliders;
-    if (noise) l.state_bytes += (uint64_t)l.Sp * 4;
-    if (flags & HS_GROOM_KEEP_CORRECTIONS) l.state_bytes += planes;
     l.pool_cap = 0;
     l.desc_cap = 0;
-    l.grid_bytes = 0;
     if (G) {
         // partial boxes: room for half a cell per point per step, at least 64k cells
         l.pool_cap = std::max<uint64_t>(65536, (uint64_t)S * (N - 1) / 2);
         l.desc_cap = 4ull * l.nwg * l.nseg + 4096;
-        l.grid_bytes = 3 * grid_cells(G) * 16                 // grid_i, grid_f, overflow
-                       + 2 * l.pool_cap * 16                  // pool halves
-                       + 2 * l.desc_cap * hs::kDescInts * 4   // descriptor halves
-                       + 8 * 4                                // counters
-                       + (uint64_t)l.nwg * l.nseg * 6 * 4;    // workgroup-segment boxes
     }
     return l;
 }
 
-struct Groom {
-    uint32_t S = 0, N = 0, G = 0, flags = 0;
-    Layout L{};
-    float origin[3] = {0, 0, 0};
-    float h = 1.0f;
-    float noise_dir[3] = {0, 0, 0};
-    float* pos = nullptr;
-    float* vel = nullptr;
-    float* corr = nullptr;
-    float* seg = nullptr;
-    float* noise = nullptr;
-    uint32_t* perm = nullptr;   // physical slot -> caller's strand index
-    hs::Collider* cols = nullptr;  // device copy of the step's colliders (ellipsoids)
-    hs::GridWork gw{};
-    std::vector<void*> allocs;  // every device allocation of this groom
-    // Grid friction of the last step, not yet applied to the stored velocities.
-    bool pending = false;
-    float pend_mu = 0.0f, pend_omu = 1.0f;
-    uint32_t parity = 0;        // pool/counter half the next splat writes
-    uint64_t bytes = 0;
-};
-
-}  // namespace
-
-struct hs_ctx {
-    int device = 0;
-    hipStream_t own = nullptr;
-    hipStream_t stream = nullptr;
-    std::string err;
-    std::map<uint32_t, Groom> grooms;
-    uint32_t next_id = 1;
-    uint64_t bytes = 0;
-    // kernel timing: pooled events (step start, step end, reduce end) per step
-    bool profile = false;
-    struct Ev {
-        hipEvent_t e[3];
-        bool reduce;
+uint64_t carve(Groom& g, uint8_t* base)
+{
+    uint64_t off = 0;
+    auto take = [&](auto*& ptr, uint64_t bytes) {
+        using T = std::remove_reference_t<decltype(*ptr)>;
+        ptr = base ? reinterpret_cast<T*>(base + off) : nullptr;
+        off += round_up(bytes, 256);
     };
-    std::vector<Ev> events;
-    size_t events_used = 0;
-    uint64_t prof_launches = 0;
-    double prof_step_ms = 0.0, prof_reduce_ms = 0.0;
-    // device staging of hs_step_batch arguments
-    hs::StepArgs* batch_args = nullptr;
-    uint32_t* batch_idx = nullptr;
-    size_t batch_args_cap = 0, batch_idx_cap = 0;
-};
+    const Layout& L = g.L;
+    const uint64_t planes = 3ull * g.N * L.Sp * 4;
+    take(g.pos, planes);
+    take(g.vel, planes);
+    if (g.flags & HS_GROOM_KEEP_CORRECTIONS)
+        take(g.corr, planes);
+    else
+        g.corr = nullptr;
+    take(g.seg, (uint64_t)L.Sp * 4);
+    take(g.perm, (uint64_t)L.Sp * 4);
+    take(g.cols, sizeof(hs::Collider) * hs::kMaxColliders);
+    if (g.has_noise)
+        take(g.noise, (uint64_t)L.Sp * 4);
+    else
+        g.noise = nullptr;
+    g.gw = hs::GridWork{};
+    if (g.G) {
+        const uint64_t cells = grid_cells(g.G);
+        take(g.gw.grid_i, cells * 16);
+        take(g.gw.grid_f, cells * 16);
+        take(g.gw.overflow, cells * 16);
+        take(g.gw.pool, 2 * L.pool_cap * 16);
+        take(g.gw.desc, 2 * L.desc_cap * hs::kDescInts * 4);
+        take(g.gw.counters, 8 * 4);
+        take(g.gw.gbox, (uint64_t)L.nwg * L.nseg * 6 * 4);
+        take(g.gw.brick_cnt, bricks(g.G) * 2 * 4);
+        take(g.gw.brick_list, bricks(g.G) * 2 * hs::kBrickList * 4);
+        take(g.gw.brick_nz, bricks(g.G) * 4);  // zero: every brick of the zeroed grid is empty
+        take(g.gw.near_limit, 8);
+        g.gw.pool_cap = (uint32_t)L.pool_cap;
+        g.gw.desc_cap = (uint32_t)L.desc_cap;
+        g.gw.nb = bricks_per_axis(g.G);
+    }
+    return off;
+}
+
+uint32_t ctx_adopt(hs_ctx* c, Groom&& g)
+{
+    c->bytes += g.bytes;
+    const uint32_t gid = c->next_id++;
+    c->grooms[gid] = std::move(g);
+    return gid;
+}
+
+bool ctx_take(hs_ctx* c, uint32_t id, Groom* out)
+{
+    auto it = c->grooms.find(id);
+    if (it == c->grooms.end()) return false;
+    c->bytes -= it->second.bytes;
+    *out = std::move(it->second);
+    c->grooms.erase(it);
+    return true;
+}
+
+void groom_release(Groom& g)
+{
+    if (g.block) {
+        if (g.heap)
+            g.heap->release(g.block);
+        else
+            (void)hipFree(g.block);
+    }
+    g = Groom();
+}
+
+}  // namespace hsi
 
 namespace {
 
@@ -131,9 +141,8 @@ void free_groom(hs_ctx* c, Groom& g)
 {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    for (void* p : g.allocs) (void)hipFree(p);
     c->bytes -= g.bytes;
-    g = Groom();
+    groom_release(g);
 }
 
 Groom* find(hs_ctx* c, uint32_t id)
@@ -158,6 +167,14 @@ int drain_profile(hs_ctx* c)
     }
     c->events_used = 0;
     return HS_OK;
+}
+
+void drop_graphs(hs_ctx* c)
+{
+    for (auto& g : c->graph_cache)
+        for (int k = 0; k < hs_ctx::kPinSlots; ++k)
+            if (g.exec[k]) (void)hipGraphExecDestroy(g.exec[k]);
+    c->graph_cache.clear();
 }
 
 // Next set of timing events from the pool (created on first use).
@@ -259,8 +276,12 @@ std::vector<uint32_t> kd_perm(const float* aos, uint32_t S, uint32_t N)
 }
 
 // Validate one step request and fill the kernel arguments for groom g.
+// `stage`: when given, ellipsoid colliders are written there (the caller
+// points a.cols at their device copy); otherwise they are uploaded to the
+// groom's collider buffer on the stream.
 int build_args(hs_ctx* c, Groom& g, const hs_time* t, const hs_collider* cols, uint32_t ncol,
-               const hs_step_params* p, const char* who, hs::StepArgs* out, bool* gather, bool* splat)
+               const hs_step_params* p, const char* who, hs::StepArgs* out, bool* gather, bool* splat,
+               hs::Collider* stage = nullptr)
 {
     const std::string w(who);
     if (!t || !p) return fail(c, HS_ERR_INVALID, w + ": null time or params");
@@ -315,12 +336,14 @@ int build_args(hs_ctx* c, Groom& g, const hs_time* t, const hs_collider* cols, u
         // ellipsoids are read from device memory; stream order keeps the
         // previous step's kernel ahead of this copy
         hs::Collider host[hs::kMaxColliders];
+        hs::Collider* dst = stage ? stage : host;
         for (uint32_t j = 0; j < a.ncol; ++j) {
-            std::memcpy(host[j].m, cols[j].transform, 64);
-            std::memcpy(host[j].m + 16, cols[j].inv, 64);
-            std::memcpy(host[j].m + 32, cols[j].invTrans, 64);
+            std::memcpy(dst[j].m, cols[j].transform, 64);
+            std::memcpy(dst[j].m + 16, cols[j].inv, 64);
+            std::memcpy(dst[j].m + 32, cols[j].invTrans, 64);
         }
-        HS_TRY(c, hipMemcpyAsync(g.cols, host, sizeof(hs::Collider) * a.ncol, hipMemcpyHostToDevice, c->stream));
+        if (!stage)
+            HS_TRY(c, hipMemcpyAsync(g.cols, host, sizeof(hs::Collider) * a.ncol, hipMemcpyHostToDevice, c->stream));
     }
     *gather = g.pending;
     if (*gather) {
@@ -342,6 +365,112 @@ void advance_state(Groom& g, bool splat, float friction)
     } else {
         g.pending = false;
     }
+}
+
+}  // namespace
+
+namespace {
+
+uint64_t align256(uint64_t x) { return round_up(x, 256); }
+
+// Make room for a blob of `bytes` in the pinned slots and on the device.
+// Growing drops the captured graphs (they hold the old addresses).
+int ensure_blob(hs_ctx* c, size_t bytes)
+{
+    if (bytes <= c->pin_cap && bytes <= c->dev_cap) return HS_OK;
+    const size_t cap = std::max<size_t>(bytes * 2, 64 * 1024);
+    HS_TRY(c, hipStreamSynchronize(c->stream));
+    drop_graphs(c);
+    for (int k = 0; k < hs_ctx::kPinSlots; ++k) {
+        if (c->pin[k]) (void)hipHostFree(c->pin[k]);
+        c->pin[k] = nullptr;
+        c->slot_used[k] = false;
+    }
+    (void)hipFree(c->dev_blob);
+    c->dev_blob = nullptr;
+    c->pin_cap = c->dev_cap = 0;
+    for (int k = 0; k < hs_ctx::kPinSlots; ++k) {
+        HS_TRY(c, hipHostMalloc((void**)&c->pin[k], cap, hipHostMallocDefault));
+        if (!c->slot_done[k]) HS_TRY(c, hipEventCreateWithFlags(&c->slot_done[k], hipEventDisableTiming));
+    }
+    HS_TRY(c, hipMalloc((void**)&c->dev_blob, cap));
+    c->pin_cap = c->dev_cap = cap;
+    return HS_OK;
+}
+
+// One kernel variant's share of a batch: which grooms, where their
+// arguments and prefix arrays sit in the blob, and the grid sizes.
+struct BatchLaunch {
+    uint64_t args_off, wg_off, br_off;
+    uint32_t ng, total_wg, total_br;
+    uint8_t v;  // bit 0 gather, bit 1 splat, bit 2 ellipsoids
+};
+
+hipError_t enqueue_batch(const hs_ctx* c, const std::vector<BatchLaunch>& launches, size_t bytes, int slot,
+                         hipStream_t st)
+{
+    hipError_t e = hipMemcpyAsync(c->dev_blob, c->pin[slot], bytes, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return e;
+    for (const BatchLaunch& L : launches) {
+        e = hs::launch_step_batch(reinterpret_cast<const hs::StepArgs*>(c->dev_blob + L.args_off),
+                                  reinterpret_cast<const uint32_t*>(c->dev_blob + L.wg_off), L.ng, L.total_wg,
+                                  (L.v & 1) != 0, (L.v & 2) != 0, (L.v & 4) != 0, st);
+        if (e != hipSuccess) return e;
+    }
+    for (const BatchLaunch& L : launches) {
+        if (!(L.v & 2)) continue;
+        e = hs::launch_reduce_batch(reinterpret_cast<const hs::StepArgs*>(c->dev_blob + L.args_off),
+                                    reinterpret_cast<const uint32_t*>(c->dev_blob + L.br_off), L.ng, L.total_br, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// The executable graph for this batch shape and pinned slot, captured on
+// first use.  At most kMaxGraphs shapes are kept (least recently used goes).
+constexpr size_t kMaxGraphs = 16;
+
+int batch_graph(hs_ctx* c, const std::vector<uint64_t>& key, const std::vector<BatchLaunch>& launches,
+                size_t bytes, int slot, hipGraphExec_t* out)
+{
+    hs_ctx::BatchGraph* hit = nullptr;
+    for (auto& g : c->graph_cache)
+        if (g.key == key) hit = &g;
+    if (!hit) {
+        if (c->graph_cache.size() >= kMaxGraphs) {
+            size_t lru = 0;
+            for (size_t k = 1; k < c->graph_cache.size(); ++k)
+                if (c->graph_cache[k].last_use < c->graph_cache[lru].last_use) lru = k;
+            for (int k = 0; k < hs_ctx::kPinSlots; ++k)
+                if (c->graph_cache[lru].exec[k]) (void)hipGraphExecDestroy(c->graph_cache[lru].exec[k]);
+            c->graph_cache.erase(c->graph_cache.begin() + (long)lru);
+        }
+        c->graph_cache.emplace_back();
+        hit = &c->graph_cache.back();
+        hit->key = key;
+    }
+    hit->last_use = c->batch_calls;
+    if (!hit->exec[slot]) {
+        if (!c->cap_stream) HS_TRY(c, hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
+        HS_TRY(c, hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeRelaxed));
+        const hipError_t le = enqueue_batch(c, launches, bytes, slot, c->cap_stream);
+        hipGraph_t graph = nullptr;
+        const hipError_t ce = hipStreamEndCapture(c->cap_stream, &graph);
+        if (le != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return hip_fail(c, le, "hs_step_batch: capture");
+        }
+        if (ce != hipSuccess) return hip_fail(c, ce, "hipStreamEndCapture");
+        const hipError_t ie = hipGraphInstantiate(&hit->exec[slot], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) {
+            hit->exec[slot] = nullptr;
+            return hip_fail(c, ie, "hipGraphInstantiate");
+        }
+        c->graph_captures += 1;
+    }
+    *out = hit->exec[slot];
+    return HS_OK;
 }
 
 }  // namespace
@@ -407,8 +536,13 @@ void hs_destroy(hs_ctx* c)
     c->grooms.clear();
     for (auto& ev : c->events)
         for (int k = 0; k < 3; ++k) (void)hipEventDestroy(ev.e[k]);
-    (void)hipFree(c->batch_args);
-    (void)hipFree(c->batch_idx);
+    drop_graphs(c);
+    for (int k = 0; k < hs_ctx::kPinSlots; ++k) {
+        if (c->pin[k]) (void)hipHostFree(c->pin[k]);
+        if (c->slot_done[k]) (void)hipEventDestroy(c->slot_done[k]);
+    }
+    (void)hipFree(c->dev_blob);
+    if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
     if (c->own) (void)hipStreamDestroy(c->own);
     delete c;
 }
@@ -441,87 +575,82 @@ int hs_memory_used(const hs_ctx* c, uint64_t* bytes)
 uint64_t hs_groom_bytes(uint32_t S, uint32_t N, uint32_t G, uint32_t flags)
 {
     if (S == 0 || N < 2) return 0;
-    const Layout l = layout_of(S, N, G, flags, true);
-    return l.state_bytes + l.grid_bytes;
+    Groom g;
+    g.S = S;
+    g.N = N;
+    g.G = G;
+    g.flags = flags;
+    g.has_noise = true;
+    g.L = layout_of(S, N, G);
+    return carve(g, nullptr);
 }
 
-int hs_groom_create(hs_ctx* c, const hs_groom_desc* d, const float* aos, uint32_t* id)
+}  // extern "C"
+
+namespace hsi {
+
+int check_groom_desc(const hs_groom_desc* d, const float* aos, std::string* why)
 {
-    if (!c) return HS_ERR_INVALID;
-    if (!d || !aos || !id) return fail(c, HS_ERR_INVALID, "hs_groom_create: null argument");
-    if (d->num_strands == 0) return fail(c, HS_ERR_INVALID, "hs_groom_create: num_strands must be >= 1");
-    if (d->num_points < 2 || d->num_points > HS_MAX_POINTS)
-        return fail(c, HS_ERR_INVALID, "hs_groom_create: num_points must be in [2, 256]");
-    if (d->num_strands > (1u << 26)) return fail(c, HS_ERR_INVALID, "hs_groom_create: too many strands");
+    auto bad = [&](const char* m) {
+        *why = m;
+        return HS_ERR_INVALID;
+    };
+    if (!d || !aos) return bad("null argument");
+    if (d->num_strands == 0) return bad("num_strands must be >= 1");
+    if (d->num_points < 2 || d->num_points > HS_MAX_POINTS) return bad("num_points must be in [2, 256]");
+    if (d->num_strands > (1u << 26)) return bad("too many strands");
     // kernels address the [3][N][Sp] state with 32-bit byte offsets: 12*N*Sp < 2^32
     if ((uint64_t)d->num_points * round_up(d->num_strands, hs::kW) * 12 >= (1ull << 32))
-        return fail(c, HS_ERR_INVALID, "hs_groom_create: more than 357M points in one groom");
-    if (d->grid_dim > 1024) return fail(c, HS_ERR_INVALID, "hs_groom_create: grid_dim > 1024");
+        return bad("more than 357M points in one groom");
+    if (d->grid_dim > 1024) return bad("grid_dim > 1024");
     if (d->grid_dim > 0 && !(d->grid_h > 0.0f && std::isfinite(d->grid_h)))
-        return fail(c, HS_ERR_INVALID, "hs_groom_create: grid_h must be finite and > 0");
-    if (!d->seg_length && !(d->strand_length > 0.0f))
-        return fail(c, HS_ERR_INVALID, "hs_groom_create: strand_length must be > 0");
-    HS_TRY(c, hipSetDevice(c->device));
+        return bad("grid_h must be finite and > 0");
+    if (!d->seg_length && !(d->strand_length > 0.0f)) return bad("strand_length must be > 0");
+    return HS_OK;
+}
 
+int groom_build(const hs_groom_desc* d, const float* aos, hipStream_t st, DevHeap* heap, Groom* out,
+                std::string* why)
+{
     Groom g;
     g.S = d->num_strands;
     g.N = d->num_points;
     g.G = d->grid_dim;
     g.flags = d->flags;
     g.h = d->grid_h;
+    g.has_noise = d->wind_noise != nullptr;
     for (int k = 0; k < 3; ++k) {
         g.origin[k] = d->grid_origin[k];
         g.noise_dir[k] = d->noise_dir[k];
     }
-    g.L = layout_of(g.S, g.N, g.G, g.flags, d->wind_noise != nullptr);
+    g.L = layout_of(g.S, g.N, g.G);
     const Layout& L = g.L;
-    const size_t planes = 3ull * g.N * L.Sp;
+    g.bytes = carve(g, nullptr);
     const size_t aos_bytes = (size_t)g.S * g.N * 48;
-    float* staging = nullptr;
-    auto cleanup = [&](int code, const std::string& msg) {
-        (void)hipStreamSynchronize(c->stream);
-        (void)hipFree(staging);
-        for (void* p : g.allocs) (void)hipFree(p);
-        return fail(c, code, msg);
+    g.heap = heap;
+    void* staging = nullptr;
+    auto alloc = [&](void** p, size_t bytes) { return heap ? heap->alloc(p, bytes) : hipMalloc(p, bytes); };
+    auto cleanup = [&](hipError_t e, const char* what) {
+        (void)hipStreamSynchronize(st);
+        if (staging) {
+            if (heap)
+                heap->release(staging);
+            else
+                (void)hipFree(staging);
+        }
+        groom_release(g);
+        *why = std::string(what) + ": " + hipGetErrorString(e);
+        return e == hipErrorOutOfMemory ? HS_ERR_OUT_OF_MEMORY : HS_ERR_HIP;
     };
-#define HS_ALLOC(ptr, nbytes)                                                                     \
-    do {                                                                                          \
-        void* p_ = nullptr;                                                                       \
-        hipError_t e_ = hipMalloc(&p_, (nbytes));                                                 \
-        if (e_ != hipSuccess)                                                                     \
-            return cleanup(e_ == hipErrorOutOfMemory ? HS_ERR_OUT_OF_MEMORY : HS_ERR_HIP,          \
-                           std::string("hipMalloc: ") + hipGetErrorString(e_));                   \
-        g.allocs.push_back(p_);                                                                   \
-        (ptr) = reinterpret_cast<decltype(ptr)>(p_);                                              \
-        e_ = hipMemsetAsync(p_, 0, (nbytes), c->stream);                                          \
-        if (e_ != hipSuccess) return cleanup(HS_ERR_HIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e_)); \
+#define HS_CHECK(expr)                                 \
+    do {                                               \
+        hipError_t e_ = (expr);                        \
+        if (e_ != hipSuccess) return cleanup(e_, #expr); \
     } while (0)
-#define HS_CHECK(expr)                                                                              \
-    do {                                                                                            \
-        hipError_t e_ = (expr);                                                                     \
-        if (e_ != hipSuccess) return cleanup(HS_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
-    } while (0)
-
-    HS_ALLOC(g.pos, planes * 4);
-    HS_ALLOC(g.vel, planes * 4);
-    if (g.flags & HS_GROOM_KEEP_CORRECTIONS) HS_ALLOC(g.corr, planes * 4);
-    HS_ALLOC(g.seg, (size_t)L.Sp * 4);
-    HS_ALLOC(g.perm, (size_t)L.Sp * 4);
-    HS_ALLOC(g.cols, sizeof(hs::Collider) * hs::kMaxColliders);
-    if (d->wind_noise) HS_ALLOC(g.noise, (size_t)L.Sp * 4);
-    if (g.G) {
-        const size_t cells = grid_cells(g.G);
-        HS_ALLOC(g.gw.grid_i, cells * 16);
-        HS_ALLOC(g.gw.grid_f, cells * 16);
-        HS_ALLOC(g.gw.overflow, cells * 16);
-        HS_ALLOC(g.gw.pool, 2 * L.pool_cap * 16);
-        HS_ALLOC(g.gw.desc, 2 * L.desc_cap * hs::kDescInts * 4);
-        HS_ALLOC(g.gw.counters, 8 * 4);
-        HS_ALLOC(g.gw.gbox, (size_t)L.nwg * L.nseg * 6 * 4);
-        g.gw.pool_cap = (uint32_t)L.pool_cap;
-        g.gw.desc_cap = (uint32_t)L.desc_cap;
-    }
-    HS_CHECK(hipMalloc((void**)&staging, aos_bytes));
+    HS_CHECK(alloc(&g.block, g.bytes));
+    carve(g, static_cast<uint8_t*>(g.block));
+    HS_CHECK(hipMemsetAsync(g.block, 0, g.bytes, st));
+    HS_CHECK(alloc(&staging, aos_bytes));
 
     // physical order: k-d leaves of the roots, one per workgroup
     const std::vector<uint32_t> perm = kd_perm(aos, g.S, g.N);
@@ -533,22 +662,56 @@ int hs_groom_create(hs_ctx* c, const hs_groom_desc* d, const float* aos, uint32_
         nz.assign(L.Sp, 0.0f);
         for (uint32_t p = 0; p < g.S; ++p) nz[p] = d->wind_noise[perm[p]];
     }
-    HS_CHECK(hipMemcpyAsync(g.seg, seg.data(), (size_t)L.Sp * 4, hipMemcpyHostToDevice, c->stream));
-    HS_CHECK(hipMemcpyAsync(g.perm, perm.data(), (size_t)g.S * 4, hipMemcpyHostToDevice, c->stream));
-    if (d->wind_noise) HS_CHECK(hipMemcpyAsync(g.noise, nz.data(), (size_t)L.Sp * 4, hipMemcpyHostToDevice, c->stream));
-    HS_CHECK(hipMemcpyAsync(staging, aos, aos_bytes, hipMemcpyHostToDevice, c->stream));
-    HS_CHECK(hs::launch_pack(staging, g.perm, g.S, g.N, L.Sp, g.pos, g.vel, g.corr, c->stream));
-    HS_CHECK(hipStreamSynchronize(c->stream));
-    HS_CHECK(hipFree(staging));
-    staging = nullptr;
-#undef HS_ALLOC
+    HS_CHECK(hipMemcpyAsync(g.seg, seg.data(), (size_t)L.Sp * 4, hipMemcpyHostToDevice, st));
+    HS_CHECK(hipMemcpyAsync(g.perm, perm.data(), (size_t)g.S * 4, hipMemcpyHostToDevice, st));
+    if (d->wind_noise) HS_CHECK(hipMemcpyAsync(g.noise, nz.data(), (size_t)L.Sp * 4, hipMemcpyHostToDevice, st));
+    HS_CHECK(hipMemcpyAsync(staging, aos, aos_bytes, hipMemcpyHostToDevice, st));
+    HS_CHECK(hs::launch_pack(static_cast<const float*>(staging), g.perm, g.S, g.N, L.Sp, g.pos, g.vel, g.corr, st));
+    HS_CHECK(hipStreamSynchronize(st));
 #undef HS_CHECK
+    if (heap)
+        heap->release(staging);
+    else
+        (void)hipFree(staging);
+    *out = std::move(g);
+    return HS_OK;
+}
 
-    g.bytes = L.state_bytes + L.grid_bytes;
-    c->bytes += g.bytes;
-    const uint32_t gid = c->next_id++;
-    c->grooms[gid] = std::move(g);
-    *id = gid;
+hipError_t groom_unpack(const Groom& g, float* aos_dev, hipStream_t st)
+{
+    hs::UnpackGrid ug;
+    const hs::UnpackGrid* pug = nullptr;
+    if (g.pending) {
+        ug.grid = g.gw.grid_i;
+        ug.G = g.G;
+        ug.ox = g.origin[0];
+        ug.oy = g.origin[1];
+        ug.oz = g.origin[2];
+        ug.inv_h = 1.0f / g.h;
+        ug.mu = g.pend_mu;
+        ug.omu = g.pend_omu;
+        pug = &ug;
+    }
+    return hs::launch_unpack(aos_dev, g.perm, g.S, g.N, g.L.Sp, g.pos, g.vel, g.corr,
+                             pug, st);
+}
+
+}  // namespace hsi
+
+extern "C" {
+
+int hs_groom_create(hs_ctx* c, const hs_groom_desc* d, const float* aos, uint32_t* id)
+{
+    if (!c) return HS_ERR_INVALID;
+    if (!id) return fail(c, HS_ERR_INVALID, "hs_groom_create: null argument");
+    std::string why;
+    int rc = check_groom_desc(d, aos, &why);
+    if (rc != HS_OK) return fail(c, rc, "hs_groom_create: " + why);
+    HS_TRY(c, hipSetDevice(c->device));
+    Groom g;
+    rc = groom_build(d, aos, c->stream, nullptr, &g, &why);
+    if (rc != HS_OK) return fail(c, rc, "hs_groom_create: " + why);
+    *id = ctx_adopt(c, std::move(g));
     return HS_OK;
 }
 
@@ -591,6 +754,7 @@ int hs_step(hs_ctx* c, uint32_t id, const hs_time* t, const hs_collider* cols, u
     return HS_OK;
 }
 
+
 int hs_step_batch(hs_ctx* c, const hs_batch_item* items, uint32_t n)
 {
     if (!c) return HS_ERR_INVALID;
@@ -602,87 +766,144 @@ int hs_step_batch(hs_ctx* c, const hs_batch_item* items, uint32_t n)
     for (uint32_t k = 0; k < n; ++k) {
         gs[k] = find(c, items[k].groom_id);
         if (!gs[k]) return fail(c, HS_ERR_NOT_FOUND, "hs_step_batch: unknown groom");
-        for (uint32_t j = 0; j < k; ++j)
-            if (gs[j] == gs[k]) return fail(c, HS_ERR_INVALID, "hs_step_batch: groom listed twice");
+        if (!items[k].params) return fail(c, HS_ERR_INVALID, "hs_step_batch: null params");
+    }
+    {
+        std::vector<uint32_t> ids(n);
+        for (uint32_t k = 0; k < n; ++k) ids[k] = items[k].groom_id;
+        std::sort(ids.begin(), ids.end());
+        if (std::adjacent_find(ids.begin(), ids.end()) != ids.end())
+            return fail(c, HS_ERR_INVALID, "hs_step_batch: groom listed twice");
     }
     std::vector<hs::StepArgs> args(n);
     std::vector<uint8_t> variant(n);
+    std::vector<hs::Collider> ell;           // ellipsoid colliders of every groom that has them
+    std::vector<uint32_t> ell_at(n, 0);
     for (uint32_t k = 0; k < n; ++k) {
         bool gather = false, splat = false;
+        hs::Collider stage[hs::kMaxColliders];
         int rc = build_args(c, *gs[k], &items[k].time, items[k].colliders, items[k].num_colliders, items[k].params,
-                            "hs_step_batch", &args[k], &gather, &splat);
+                            "hs_step_batch", &args[k], &gather, &splat, stage);
         if (rc != HS_OK) return rc;
-        variant[k] = (uint8_t)((gather ? 1 : 0) | (splat ? 2 : 0) | (args[k].ncol > 1 ? 4 : 0));
+        const bool e = args[k].ncol > 1;
+        variant[k] = (uint8_t)((gather ? 1 : 0) | (splat ? 2 : 0) | (e ? 4 : 0));
+        if (e) {
+            ell_at[k] = (uint32_t)ell.size();
+            ell.insert(ell.end(), stage, stage + args[k].ncol);
+        }
     }
-    // one launch pair per kernel variant present (usually exactly one)
-    hs_ctx::Ev* ev = nullptr;
-    bool any_splat = false;
-    for (uint32_t k = 0; k < n; ++k) any_splat |= (variant[k] & 2) != 0;
-    if (c->profile) {
-        int rc = next_events(c, &ev);
-        if (rc != HS_OK) return rc;
-        ev->reduce = any_splat;
-        HS_TRY(c, hipEventRecord(ev->e[0], c->stream));
-    }
+    // blob: [StepArgs of every group][ellipsoids][prefix arrays of every group]
     std::vector<std::vector<uint32_t>> groups(8);
     for (uint32_t k = 0; k < n; ++k) groups[variant[k]].push_back(k);
-    // host staging: args of every group, then the prefix arrays
-    std::vector<hs::StepArgs> hargs;
-    std::vector<uint32_t> hidx;
-    struct Launch {
-        size_t args_off, wg_off, br_off;
-        uint32_t ng, total_wg, total_br;
-        uint8_t v;
-    };
-    std::vector<Launch> launches;
+    std::vector<uint32_t> order;
+    std::vector<uint32_t> idx;
+    std::vector<BatchLaunch> launches;
+    std::vector<uint64_t> key;
     for (uint8_t v = 0; v < 8; ++v) {
         if (groups[v].empty()) continue;
-        Launch L{hargs.size(), 0, 0, (uint32_t)groups[v].size(), 0, 0, v};
+        BatchLaunch L{order.size() * sizeof(hs::StepArgs), 0, 0, (uint32_t)groups[v].size(), 0, 0, v};
         std::vector<uint32_t> wg{0}, br{0};
+        key.push_back(((uint64_t)v << 32) | groups[v].size());
         for (uint32_t k : groups[v]) {
-            hargs.push_back(args[k]);
+            order.push_back(k);
+            key.push_back(((uint64_t)items[k].groom_id << 8) | args[k].ncol);
             wg.push_back(wg.back() + args[k].nwg);
             const uint32_t nb = args[k].G ? (args[k].G + hs::kBrick - 1) / hs::kBrick : 0;
             br.push_back(br.back() + ((v & 2) ? nb * nb * nb : 0));
         }
         L.total_wg = wg.back();
         L.total_br = br.back();
-        L.wg_off = hidx.size();
-        hidx.insert(hidx.end(), wg.begin(), wg.end());
-        L.br_off = hidx.size();
-        hidx.insert(hidx.end(), br.begin(), br.end());
+        L.wg_off = idx.size();
+        idx.insert(idx.end(), wg.begin(), wg.end());
+        L.br_off = idx.size();
+        idx.insert(idx.end(), br.begin(), br.end());
         launches.push_back(L);
     }
-    const size_t need_args = hargs.size() * sizeof(hs::StepArgs);
-    const size_t need_idx = hidx.size() * sizeof(uint32_t);
-    if (need_args > c->batch_args_cap || need_idx > c->batch_idx_cap) {
-        HS_TRY(c, hipStreamSynchronize(c->stream));
-        (void)hipFree(c->batch_args);
-        (void)hipFree(c->batch_idx);
-        c->batch_args = nullptr;
-        c->batch_idx = nullptr;
-        c->batch_args_cap = c->batch_idx_cap = 0;
-        HS_TRY(c, hipMalloc((void**)&c->batch_args, std::max<size_t>(need_args, 4096) * 2));
-        HS_TRY(c, hipMalloc((void**)&c->batch_idx, std::max<size_t>(need_idx, 4096) * 2));
-        c->batch_args_cap = std::max<size_t>(need_args, 4096) * 2;
-        c->batch_idx_cap = std::max<size_t>(need_idx, 4096) * 2;
+    const uint64_t args_bytes = align256(order.size() * sizeof(hs::StepArgs));
+    const uint64_t ell_bytes = align256(ell.size() * sizeof(hs::Collider));
+    const uint64_t bytes = args_bytes + ell_bytes + idx.size() * sizeof(uint32_t);
+    for (BatchLaunch& L : launches) {
+        L.wg_off = args_bytes + ell_bytes + L.wg_off * sizeof(uint32_t);
+        L.br_off = args_bytes + ell_bytes + L.br_off * sizeof(uint32_t);
     }
-    // pageable sources: the copies are staged before returning, and stream
-    // order keeps the previous batch's kernels ahead of them
-    HS_TRY(c, hipMemcpyAsync(c->batch_args, hargs.data(), need_args, hipMemcpyHostToDevice, c->stream));
-    HS_TRY(c, hipMemcpyAsync(c->batch_idx, hidx.data(), need_idx, hipMemcpyHostToDevice, c->stream));
-    for (const Launch& L : launches) {
-        const hs::StepArgs* ad = c->batch_args + L.args_off;
-        HS_TRY(c, hs::launch_step_batch(ad, c->batch_idx + L.wg_off, L.ng, L.total_wg, (L.v & 1) != 0,
-                                        (L.v & 2) != 0, (L.v & 4) != 0, c->stream));
+    key.push_back(bytes);
+    int rc = ensure_blob(c, bytes);
+    if (rc != HS_OK) return rc;
+    const int slot = c->slot_next;
+    c->slot_next = (slot + 1) % hs_ctx::kPinSlots;
+    if (c->slot_used[slot]) HS_TRY(c, hipEventSynchronize(c->slot_done[slot]));
+    uint8_t* h = c->pin[slot];
+    hs::Collider* ell_dev = reinterpret_cast<hs::Collider*>(c->dev_blob + args_bytes);
+    for (size_t j = 0; j < order.size(); ++j) {
+        hs::StepArgs a = args[order[j]];
+        if (a.ncol > 1) a.cols = ell_dev + ell_at[order[j]];
+        std::memcpy(h + j * sizeof(hs::StepArgs), &a, sizeof(a));
     }
-    if (ev) HS_TRY(c, hipEventRecord(ev->e[1], c->stream));
-    for (const Launch& L : launches)
-        if (L.v & 2)
-            HS_TRY(c, hs::launch_reduce_batch(c->batch_args + L.args_off, c->batch_idx + L.br_off, L.ng,
-                                              L.total_br, c->stream));
-    if (ev && any_splat) HS_TRY(c, hipEventRecord(ev->e[2], c->stream));
+    if (!ell.empty()) std::memcpy(h + args_bytes, ell.data(), ell.size() * sizeof(hs::Collider));
+    std::memcpy(h + args_bytes + ell_bytes, idx.data(), idx.size() * sizeof(uint32_t));
+
+    hipGraphExec_t exec = nullptr;
+    if (c->graphs) {
+        rc = batch_graph(c, key, launches, bytes, slot, &exec);
+        if (rc != HS_OK) return rc;
+    }
+    hs_ctx::Ev* ev = nullptr;
+    bool any_splat = false;
+    for (uint32_t k = 0; k < n; ++k) any_splat |= (variant[k] & 2) != 0;
+    if (c->profile) {
+        rc = next_events(c, &ev);
+        if (rc != HS_OK) return rc;
+        ev->reduce = false;
+    }
+    if (exec) {
+        if (ev) HS_TRY(c, hipEventRecord(ev->e[0], c->stream));
+        HS_TRY(c, hipGraphLaunch(exec, c->stream));
+        if (ev) HS_TRY(c, hipEventRecord(ev->e[1], c->stream));
+    } else if (ev) {
+        // eager launches: split the timing into step kernels and reduce kernels
+        HS_TRY(c, hipMemcpyAsync(c->dev_blob, c->pin[slot], bytes, hipMemcpyHostToDevice, c->stream));
+        HS_TRY(c, hipEventRecord(ev->e[0], c->stream));
+        for (const BatchLaunch& L : launches)
+            HS_TRY(c, hs::launch_step_batch(reinterpret_cast<const hs::StepArgs*>(c->dev_blob + L.args_off),
+                                            reinterpret_cast<const uint32_t*>(c->dev_blob + L.wg_off), L.ng,
+                                            L.total_wg, (L.v & 1) != 0, (L.v & 2) != 0, (L.v & 4) != 0, c->stream));
+        HS_TRY(c, hipEventRecord(ev->e[1], c->stream));
+        for (const BatchLaunch& L : launches)
+            if (L.v & 2)
+                HS_TRY(c, hs::launch_reduce_batch(reinterpret_cast<const hs::StepArgs*>(c->dev_blob + L.args_off),
+                                                  reinterpret_cast<const uint32_t*>(c->dev_blob + L.br_off), L.ng,
+                                                  L.total_br, c->stream));
+        if (any_splat) {
+            ev->reduce = true;
+            HS_TRY(c, hipEventRecord(ev->e[2], c->stream));
+        }
+    } else {
+        HS_TRY(c, enqueue_batch(c, launches, bytes, slot, c->stream));
+    }
+    HS_TRY(c, hipEventRecord(c->slot_done[slot], c->stream));
+    c->slot_used[slot] = true;
+    c->batch_calls += 1;
     for (uint32_t k = 0; k < n; ++k) advance_state(*gs[k], (variant[k] & 2) != 0, items[k].params->friction);
+    return HS_OK;
+}
+
+int hs_set_batch_graphs(hs_ctx* c, int on)
+{
+    if (!c) return HS_ERR_INVALID;
+    HS_TRY(c, hipSetDevice(c->device));
+    if (!on) {
+        HS_TRY(c, hipStreamSynchronize(c->stream));
+        drop_graphs(c);
+    }
+    c->graphs = on != 0;
+    return HS_OK;
+}
+
+int hs_batch_stats(const hs_ctx* c, uint64_t* batch_calls, uint64_t* graph_captures)
+{
+    if (!c) return HS_ERR_INVALID;
+    if (batch_calls) *batch_calls = c->batch_calls;
+    if (graph_captures) *graph_captures = c->graph_captures;
     return HS_OK;
 }
 
@@ -703,20 +924,7 @@ int hs_read_strands(hs_ctx* c, uint32_t id, float* aos, hs_draw_indirect* ind)
     const size_t bytes = (size_t)g->S * g->N * 48;
     float* dev = nullptr;
     HS_TRY(c, hipMalloc((void**)&dev, bytes));
-    hs::UnpackGrid ug;
-    const hs::UnpackGrid* pug = nullptr;
-    if (g->pending) {
-        ug.grid = g->gw.grid_i;
-        ug.G = g->G;
-        ug.ox = g->origin[0];
-        ug.oy = g->origin[1];
-        ug.oz = g->origin[2];
-        ug.inv_h = 1.0f / g->h;
-        ug.mu = g->pend_mu;
-        ug.omu = g->pend_omu;
-        pug = &ug;
-    }
-    hipError_t e = hs::launch_unpack(dev, g->perm, g->S, g->N, g->L.Sp, g->pos, g->vel, g->corr, pug, c->stream);
+    hipError_t e = groom_unpack(*g, dev, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(aos, dev, bytes, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(dev);
@@ -758,6 +966,22 @@ int hs_read_grid(hs_ctx* c, uint32_t id, hs_grid_cell* cells)
     }
     HS_TRY(c, hipMemcpyAsync(cells, g->gw.grid_i, bytes, hipMemcpyDeviceToHost, c->stream));
     HS_TRY(c, hipStreamSynchronize(c->stream));
+    return HS_OK;
+}
+
+int hs_grid_near_limit(hs_ctx* c, uint32_t id, uint64_t* cells)
+{
+    if (!c) return HS_ERR_INVALID;
+    if (!cells) return fail(c, HS_ERR_INVALID, "hs_grid_near_limit: null output");
+    Groom* g = find(c, id);
+    if (!g) return fail(c, HS_ERR_NOT_FOUND, "hs_grid_near_limit: unknown groom");
+    *cells = 0;
+    if (g->G == 0) return HS_OK;
+    HS_TRY(c, hipSetDevice(c->device));
+    unsigned long long v = 0;
+    HS_TRY(c, hipMemcpyAsync(&v, g->gw.near_limit, 8, hipMemcpyDeviceToHost, c->stream));
+    HS_TRY(c, hipStreamSynchronize(c->stream));
+    *cells = v;
     return HS_OK;
 }
 

@@ -8,12 +8,27 @@
 
 namespace hs {
 
-constexpr uint32_t kW = 128;          // strands per step workgroup (2 wavefronts)
-constexpr uint32_t kL = 16;           // points per segment of the root-to-tip sweep
-constexpr uint32_t kWinCells = 2048;  // LDS voxel window of one workgroup (16 B per cell)
+// Tile shape of the step kernel (compile-time tunables; the defaults are the
+// measured best on MI355X for the 100k x 32 workload).
+#ifndef HS_TILE_STRANDS
+#define HS_TILE_STRANDS 128
+#endif
+#ifndef HS_TILE_POINTS
+#define HS_TILE_POINTS 16
+#endif
+#ifndef HS_WINDOW_CELLS
+#define HS_WINDOW_CELLS 2048
+#endif
+#ifndef HS_VOXEL_COPIES
+#define HS_VOXEL_COPIES 1
+#endif
+constexpr uint32_t kW = HS_TILE_STRANDS;         // strands per step workgroup
+constexpr uint32_t kL = HS_TILE_POINTS;          // points per segment of the root-to-tip sweep
+constexpr uint32_t kWinCells = HS_WINDOW_CELLS;  // LDS window of one workgroup, 16 B entries
 constexpr uint32_t kBrick = 8;        // reduce kernel: 8x8x8 voxels per workgroup
 constexpr uint32_t kMaxColliders = 8;
 constexpr uint32_t kDescInts = 8;     // partial-box descriptor: lo xyz, ext xyz, pool offset, pad
+constexpr uint32_t kBrickList = 512;  // box indices kept per brick (more: the brick scans all boxes)
 
 // Device-side grid workspace of one groom (all zeroed at creation).
 struct GridWork {
@@ -24,7 +39,11 @@ struct GridWork {
     uint32_t* counters;   // [2][4]: cells used, boxes used, overflow flag, pad
     int4* overflow;       // G^3 int4 atomic fallback when the pool is full (normally zero)
     int* gbox;            // [nwg][nseg][6] voxel box each workgroup-segment touched last step
-    uint32_t pool_cap, desc_cap;
+    uint32_t* brick_cnt;  // [2][nbricks] boxes listed per 8^3 brick (may exceed kBrickList)
+    uint32_t* brick_list; // [2][nbricks][kBrickList] indices of the boxes overlapping a brick
+    uint32_t* brick_nz;   // [nbricks] 1 when the brick's grid cells are not all zero
+    unsigned long long* near_limit;  // (cell, step) pairs with a component >= 2^30 in magnitude
+    uint32_t pool_cap, desc_cap, nb;  // nb: bricks per axis
 };
 
 // Everything one step launch needs, passed by value as the kernel argument.

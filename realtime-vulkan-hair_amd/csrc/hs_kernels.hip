@@ -162,9 +162,12 @@ __device__ __forceinline__ f3 gather_block(const float4* win, const Win& wd, con
 }
 
 // Any non-empty block (clipped at the grid boundary, or outside the window):
-// from the LDS window when `lds`, else from the global float grid.
-__device__ __forceinline__ f3 gather_any(const float4* win, const Win& wd, bool lds, const float4* __restrict__ g,
-                                         const Rng& r, f3 q, int G)
+// from the LDS window when `lds`, else from the global float grid.  The two
+// sources stay in separate loops: a pointer selected between LDS and global
+// memory would be a generic pointer.
+template <bool LDS>
+__device__ __forceinline__ f3 gather_any_src(const float4* win, const Win& wd, const float4* __restrict__ g,
+                                             const Rng& r, f3 q, int G)
 {
     f3 gv = mk(0.0f, 0.0f, 0.0f);
 #pragma unroll 1
@@ -176,13 +179,22 @@ __device__ __forceinline__ f3 gather_any(const float4* win, const Win& wd, bool 
 #pragma unroll 1
             for (int c = r.z0; c <= r.z1; ++c) {
                 const float w = wxy * wgt(q.z, c);
-                const float4 cell =
-                    lds ? win[win_index(wd, a, b, c)] : g[(size_t)a + (size_t)b * G + (size_t)c * G * G];
+                float4 cell;
+                if (LDS)
+                    cell = win[win_index(wd, a, b, c)];
+                else
+                    cell = g[(uint32_t)a + (uint32_t)b * (uint32_t)G + (uint32_t)c * (uint32_t)G * (uint32_t)G];
                 gather_cell(gv, cell, w);
             }
         }
     }
     return gv;
+}
+
+__device__ __forceinline__ f3 gather_any(const float4* win, const Win& wd, bool lds, const float4* __restrict__ g,
+                                         const Rng& r, f3 q, int G)
+{
+    return lds ? gather_any_src<true>(win, wd, g, r, q, G) : gather_any_src<false>(win, wd, g, r, q, G);
 }
 
 // From the integer grid, literally as compute.comp:274-293 (read-back path).
@@ -215,7 +227,8 @@ __device__ __forceinline__ f3 gather_global_i(const int4* __restrict__ g, const 
 // Density addends are in [0, 1e6] and one window holds at most kW*(kL+1)
 // points, so the low word never carries into vx: both sums are exact mod 2^32.
 // ---------------------------------------------------------------------------
-constexpr int kSlots = 2;
+constexpr int kSlots = HS_VOXEL_COPIES;
+static_assert((kSlots & (kSlots - 1)) == 0, "copies per voxel: a power of two");
 constexpr int kSplatCells = (int)kWinCells / kSlots;  // voxels per splat window
 
 __device__ __forceinline__ void splat_cell(unsigned char* cellp, float w, f3 v)
@@ -484,7 +497,7 @@ __device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
 {
     __shared__ __align__(16) int4 win[kWinCells];
     __shared__ int red[kWaves * 6];
-    __shared__ uint32_t bcast[2];
+    __shared__ uint32_t bcast[3];
 
     const uint32_t s = wg * kW + threadIdx.x;
     const bool active = s < a.S;
@@ -759,10 +772,27 @@ __device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
                             if (!fits) atomicOr(ctr + 2, 1u);
                             bcast[0] = off;
                             bcast[1] = fits ? 1u : 0u;
+                            bcast[2] = di;
                         }
                         __syncthreads();
                         const uint32_t off = bcast[0];
                         const bool fits = bcast[1] != 0;
+                        if (fits) {
+                            // list the box with every brick it overlaps (one device atomic each, in parallel)
+                            const uint32_t di = bcast[2];
+                            const int bx0 = cw.x0 / (int)kBrick, by0 = cw.y0 / (int)kBrick, bz0 = cw.z0 / (int)kBrick;
+                            const int nbx = (cw.x0 + cw.ex - 1) / (int)kBrick - bx0 + 1;
+                            const int nby = (cw.y0 + cw.ey - 1) / (int)kBrick - by0 + 1;
+                            const int nbz = (cw.z0 + cw.ez - 1) / (int)kBrick - bz0 + 1;
+                            const uint32_t nbr = a.gw.nb * a.gw.nb * a.gw.nb;
+                            for (int t = threadIdx.x; t < nbx * nby * nbz; t += kW) {
+                                const uint32_t b = ((uint32_t)(bz0 + t / (nbx * nby)) * a.gw.nb +
+                                                    (uint32_t)(by0 + (t / nbx) % nby)) * a.gw.nb +
+                                                   (uint32_t)(bx0 + t % nbx);
+                                const uint32_t at = atomicAdd(a.gw.brick_cnt + a.parity * nbr + b, 1u);
+                                if (at < kBrickList) a.gw.brick_list[((size_t)a.parity * nbr + b) * kBrickList + at] = di;
+                            }
+                        }
                         int4* dst = a.gw.pool + (size_t)a.parity * a.gw.pool_cap + off;
                         for (int c = threadIdx.x; c < cells; c += kW) {
                             // sum the voxel's copies: (u64 vx<<32 | density), vy, vz
@@ -846,15 +876,61 @@ __global__ __launch_bounds__(kW) void k_step_batch(const StepArgs* __restrict__ 
 constexpr uint32_t kReduceThreads = 256;
 constexpr uint32_t kListCap = 4096;
 
+// Accumulate the cells of partial box `d` that fall in the brick into the LDS
+// brick accumulator; each lane takes up to kPerLane cells, loads first.
+constexpr int kRCells = kBrick * kBrick * kBrick;
+constexpr int kPerLane = kRCells / 64;
+
+struct BoxLoads {
+    int4 v[kPerLane];
+    int li[kPerLane];
+};
+
+__device__ __forceinline__ void box_loads(BoxLoads& L, const int* __restrict__ d, const int4* __restrict__ pool,
+                                          int x0, int y0, int z0, int x1, int y1, int z1, int lane)
+{
+    const int dx = d[0], dy = d[1], dz = d[2], ex = d[3], ey = d[4], off = d[6];
+    const int ix0 = max(dx, x0), iy0 = max(dy, y0), iz0 = max(dz, z0);
+    const int nx = min(dx + ex - 1, x1) - ix0 + 1;
+    const int ny = min(dy + ey - 1, y1) - iy0 + 1;
+    const int nz = min(dz + d[5] - 1, z1) - iz0 + 1;
+    const int ncell = (ex > 0 && nx > 0 && ny > 0 && nz > 0) ? nx * ny * nz : 0;
+#pragma unroll
+    for (int k = 0; k < kPerLane; ++k) {
+        const int c = lane + 64 * k;
+        L.li[k] = -1;
+        L.v[k] = make_int4(0, 0, 0, 0);
+        if (c < ncell) {
+            const int cx = ix0 + c % nx, cy = iy0 + (c / nx) % ny, cz = iz0 + c / (nx * ny);
+            L.v[k] = pool[(uint32_t)off + ((uint32_t)(cz - dz) * (uint32_t)ey + (uint32_t)(cy - dy)) * (uint32_t)ex +
+                          (uint32_t)(cx - dx)];
+            L.li[k] = ((cz - z0) * (int)kBrick + (cy - y0)) * (int)kBrick + (cx - x0);
+        }
+    }
+}
+
+__device__ __forceinline__ void box_adds(const BoxLoads& L, uint32_t (*acc)[4])
+{
+#pragma unroll
+    for (int k = 0; k < kPerLane; ++k) {
+        if (L.li[k] < 0) continue;
+        atomicAdd(&acc[L.li[k]][0], (uint32_t)L.v[k].x);
+        atomicAdd(&acc[L.li[k]][1], (uint32_t)L.v[k].y);
+        atomicAdd(&acc[L.li[k]][2], (uint32_t)L.v[k].z);
+        atomicAdd(&acc[L.li[k]][3], (uint32_t)L.v[k].w);
+    }
+}
+
 __device__ __forceinline__ void reduce_body(const GridWork& gw, uint32_t G, uint32_t parity, uint32_t brick,
                                             bool reset_counters)
 {
-    constexpr int kCells = kBrick * kBrick * kBrick;
     constexpr int kRWaves = kReduceThreads / 64;
     __shared__ int list[kListCap];
     __shared__ uint32_t nlist;
-    __shared__ uint32_t acc[kCells][4];
-    const uint32_t nb = (G + kBrick - 1) / kBrick;
+    __shared__ uint32_t acc[kRCells][4];
+    __shared__ uint32_t any_nz;
+    const uint32_t nb = gw.nb;
+    const uint32_t nbr = nb * nb * nb;
     const uint32_t bx = brick % nb, by = (brick / nb) % nb, bz = brick / (nb * nb);
     const int x0 = bx * kBrick, y0 = by * kBrick, z0 = bz * kBrick;
     const int x1 = min(x0 + (int)kBrick, (int)G) - 1, y1 = min(y0 + (int)kBrick, (int)G) - 1,
@@ -864,64 +940,67 @@ __device__ __forceinline__ void reduce_body(const GridWork& gw, uint32_t G, uint
     const int* desc = gw.desc + (size_t)parity * gw.desc_cap * kDescInts;
     const int4* pool = gw.pool + (size_t)parity * gw.pool_cap;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const bool ovf = ctr[2] != 0;
+    const uint32_t nlisted = gw.brick_cnt[parity * nbr + brick];
+    const uint32_t* blist = gw.brick_list + ((size_t)parity * nbr + brick) * kBrickList;
 
-    for (int c = threadIdx.x; c < kCells; c += kReduceThreads) {
+    if (threadIdx.x == 0) gw.brick_cnt[(parity ^ 1u) * nbr + brick] = 0;  // for the step after next
+    if (reset_counters && threadIdx.x == 0) {
+        uint32_t* nxt = gw.counters + (parity ^ 1u) * 4;
+        nxt[0] = 0;
+        nxt[1] = 0;
+        nxt[2] = 0;
+    }
+    // nothing listed, no overflow, and zero already: the brick stays as it is
+    if (nlisted == 0 && !ovf && gw.brick_nz[brick] == 0) return;
+
+    for (int c = threadIdx.x; c < kRCells; c += kReduceThreads) {
         acc[c][0] = 0;
         acc[c][1] = 0;
         acc[c][2] = 0;
         acc[c][3] = 0;
     }
-    for (uint32_t base = 0; base < ndesc; base += kListCap) {
-        if (threadIdx.x == 0) nlist = 0;
+    if (threadIdx.x == 0) any_nz = 0;
+    if (nlisted <= kBrickList) {
         __syncthreads();
-        const uint32_t end = min(base + kListCap, ndesc);
-        for (uint32_t i = base + threadIdx.x; i < end; i += kReduceThreads) {
-            const int* d = desc + (size_t)i * kDescInts;
-            const int dx1 = d[0] + d[3] - 1, dy1 = d[1] + d[4] - 1, dz1 = d[2] + d[5] - 1;
-            if (d[3] > 0 && d[0] <= x1 && dx1 >= x0 && d[1] <= y1 && dy1 >= y0 && d[2] <= z1 && dz1 >= z0)
-                list[atomicAdd(&nlist, 1u)] = (int)i;
+        // each wavefront takes two boxes at a time: 2 x kPerLane loads in flight per lane
+        for (uint32_t j = 2 * wave; j < nlisted; j += 2 * kRWaves) {
+            BoxLoads A, B;
+            box_loads(A, desc + (size_t)blist[j] * kDescInts, pool, x0, y0, z0, x1, y1, z1, lane);
+            const bool two = j + 1 < nlisted;
+            if (two) box_loads(B, desc + (size_t)blist[j + 1] * kDescInts, pool, x0, y0, z0, x1, y1, z1, lane);
+            box_adds(A, acc);
+            if (two) box_adds(B, acc);
         }
-        __syncthreads();
-        const uint32_t n = nlist;
-        // each wavefront takes whole partial boxes; each lane loads up to 8 of
-        // the box-brick intersection's cells at once, then adds them in LDS
-        for (uint32_t j = wave; j < n; j += kRWaves) {
-            const int* d = desc + (size_t)list[j] * kDescInts;
-            const int dx = d[0], dy = d[1], dz = d[2], ex = d[3], ey = d[4], off = d[6];
-            const int ix0 = max(dx, x0), iy0 = max(dy, y0), iz0 = max(dz, z0);
-            const int nx = min(dx + ex - 1, x1) - ix0 + 1;
-            const int ny = min(dy + ey - 1, y1) - iy0 + 1;
-            const int nz = min(dz + d[5] - 1, z1) - iz0 + 1;
-            const int ncell = nx * ny * nz;
-            int4 v[kCells / 64];
-            int li[kCells / 64];
-#pragma unroll
-            for (int k = 0; k < kCells / 64; ++k) {
-                const int c = lane + 64 * k;
-                li[k] = -1;
-                v[k] = make_int4(0, 0, 0, 0);
-                if (c < ncell) {
-                    const int cx = ix0 + c % nx, cy = iy0 + (c / nx) % ny, cz = iz0 + c / (nx * ny);
-                    v[k] = pool[(size_t)off + ((size_t)(cz - dz) * ey + (cy - dy)) * ex + (cx - dx)];
-                    li[k] = ((cz - z0) * kBrick + (cy - y0)) * kBrick + (cx - x0);
-                }
+    } else {
+        // the brick's list overflowed: scan every box of the step
+        for (uint32_t base = 0; base < ndesc; base += kListCap) {
+            if (threadIdx.x == 0) nlist = 0;
+            __syncthreads();
+            const uint32_t end = min(base + kListCap, ndesc);
+            for (uint32_t i = base + threadIdx.x; i < end; i += kReduceThreads) {
+                const int* d = desc + (size_t)i * kDescInts;
+                const int dx1 = d[0] + d[3] - 1, dy1 = d[1] + d[4] - 1, dz1 = d[2] + d[5] - 1;
+                if (d[3] > 0 && d[0] <= x1 && dx1 >= x0 && d[1] <= y1 && dy1 >= y0 && d[2] <= z1 && dz1 >= z0)
+                    list[atomicAdd(&nlist, 1u)] = (int)i;
             }
-#pragma unroll
-            for (int k = 0; k < kCells / 64; ++k) {
-                if (li[k] < 0) continue;
-                atomicAdd(&acc[li[k]][0], (uint32_t)v[k].x);
-                atomicAdd(&acc[li[k]][1], (uint32_t)v[k].y);
-                atomicAdd(&acc[li[k]][2], (uint32_t)v[k].z);
-                atomicAdd(&acc[li[k]][3], (uint32_t)v[k].w);
+            __syncthreads();
+            const uint32_t n = nlist;
+            for (uint32_t j = wave; j < n; j += kRWaves) {
+                BoxLoads A;
+                box_loads(A, desc + (size_t)list[j] * kDescInts, pool, x0, y0, z0, x1, y1, z1, lane);
+                box_adds(A, acc);
             }
+            __syncthreads();
         }
-        __syncthreads();
     }
-    const bool ovf = ctr[2] != 0;
-    for (int c = threadIdx.x; c < kCells; c += kReduceThreads) {
+    __syncthreads();
+    bool nz = false;
+    uint32_t near = 0;
+    for (int c = threadIdx.x; c < kRCells; c += kReduceThreads) {
         const int cx = x0 + c % kBrick, cy = y0 + (c / kBrick) % kBrick, cz = z0 + c / (kBrick * kBrick);
         if (cx > x1 || cy > y1 || cz > z1) continue;
-        const size_t gi = (size_t)cx + (size_t)cy * G + (size_t)cz * G * G;
+        const uint32_t gi = (uint32_t)cx + (uint32_t)cy * G + (uint32_t)cz * G * G;
         int4 v = make_int4((int)acc[c][0], (int)acc[c][1], (int)acc[c][2], (int)acc[c][3]);
         if (ovf) {
             const int4 o = gw.overflow[gi];
@@ -931,15 +1010,16 @@ __device__ __forceinline__ void reduce_body(const GridWork& gw, uint32_t G, uint
             v.w = (int)((uint32_t)v.w + (uint32_t)o.w);
             gw.overflow[gi] = make_int4(0, 0, 0, 0);
         }
+        nz |= (v.x | v.y | v.z | v.w) != 0;
+        near += (v.x >= (1 << 30) || v.x <= -(1 << 30) || v.y >= (1 << 30) || v.y <= -(1 << 30) ||
+                 v.z >= (1 << 30) || v.z <= -(1 << 30) || v.w >= (1 << 30) || v.w <= -(1 << 30)) ? 1u : 0u;
         gw.grid_i[gi] = v;
         gw.grid_f[gi] = make_float4((float)v.x, (float)v.y, (float)v.z, v.w > 0 ? 1.0f / (float)v.w : 0.0f);
     }
-    if (reset_counters && threadIdx.x == 0) {
-        uint32_t* nxt = gw.counters + (parity ^ 1u) * 4;
-        nxt[0] = 0;
-        nxt[1] = 0;
-        nxt[2] = 0;
-    }
+    if (nz) any_nz = 1;
+    if (near) atomicAdd(gw.near_limit, (unsigned long long)near);
+    __syncthreads();
+    if (threadIdx.x == 0) gw.brick_nz[brick] = any_nz;
 }
 
 __global__ __launch_bounds__(kReduceThreads) void k_reduce(GridWork gw, uint32_t G, uint32_t parity)

@@ -20,7 +20,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libhairsim.so")
+LIB_PATH = os.environ.get("HAIRSIM_LIB") or os.path.join(_HERE, "libhairsim.so")
 
 HS_OK = 0
 HS_ERR_INVALID = -1
@@ -109,6 +109,38 @@ class HsBatchItem(C.Structure):
     ]
 
 
+class HsKeyframe(C.Structure):
+    _fields_ = [("t", C.c_float), ("wind", C.c_float * 3), ("sphere", C.c_float * 3)]
+
+
+class HsJob(C.Structure):
+    _fields_ = [
+        ("groom", HsGroomDesc),
+        ("strands_aos", C.POINTER(C.c_float)),
+        ("steps", C.c_uint32),
+        ("dt", C.c_float),
+        ("keys", C.POINTER(HsKeyframe)),
+        ("num_keys", C.c_uint32),
+        ("params", HsStepParams),
+        ("result_aos", C.POINTER(C.c_float)),
+    ]
+
+
+class HsServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("hbm_budget", C.c_uint64), ("max_points_per_step", C.c_uint64),
+                ("use_graphs", C.c_int)]
+
+
+class HsServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64), ("admitted", C.c_uint64), ("completed", C.c_uint64), ("cancelled", C.c_uint64),
+        ("launches", C.c_uint64), ("job_steps", C.c_uint64), ("point_updates", C.c_uint64),
+        ("p50_step_ms", C.c_double), ("p99_step_ms", C.c_double), ("max_step_ms", C.c_double),
+        ("p50_wait_ms", C.c_double), ("p99_wait_ms", C.c_double), ("busy_s", C.c_double),
+        ("peak_resident_bytes", C.c_uint64), ("peak_jobs_per_launch", C.c_uint32),
+    ]
+
+
 # Every symbol include/hairsim.h declares, with its ctypes signature.
 _P = C.c_void_p
 _SIGS = {
@@ -129,12 +161,25 @@ _SIGS = {
     "hs_read_strands": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_float), C.POINTER(HsDrawIndirect)]),
     "hs_write_strands": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_float)]),
     "hs_read_grid": (C.c_int, [_P, C.c_uint32, C.POINTER(HsGridCell)]),
+    "hs_grid_near_limit": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint64)]),
+    "hs_set_batch_graphs": (C.c_int, [_P, C.c_int]),
+    "hs_batch_stats": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "hs_profile_enable": (C.c_int, [_P, C.c_int]),
     "hs_profile_read": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
     "hs_profile_breakdown": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "hs_collider_from_trs": (None, [C.POINTER(HsCollider), C.POINTER(C.c_float), C.POINTER(C.c_float),
                                     C.POINTER(C.c_float)]),
     "hs_collider_translate": (None, [C.POINTER(HsCollider), C.POINTER(C.c_float)]),
+    "hs_server_create": (C.c_int, [C.POINTER(HsServerConfig), C.POINTER(_P)]),
+    "hs_server_submit": (C.c_int, [_P, C.POINTER(HsJob), C.POINTER(C.c_uint64)]),
+    "hs_server_wait": (C.c_int, [_P, C.c_uint64, C.c_double]),
+    "hs_server_cancel_pending": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    "hs_server_drain": (C.c_int, [_P]),
+    "hs_server_get_stats": (C.c_int, [_P, C.POINTER(HsServerStats)]),
+    "hs_server_error": (C.c_int, [_P, C.POINTER(C.c_char_p)]),
+    "hs_server_destroy": (None, [_P]),
+    "hs_keyframes_eval": (None, [C.POINTER(HsKeyframe), C.c_uint32, C.c_float, C.POINTER(C.c_float),
+                                 C.POINTER(C.c_float)]),
 }
 
 _lib = None
@@ -272,6 +317,18 @@ class Context:
             items[k].params = C.pointer(params)
         self._check(lib().hs_step_batch(self._h, items, len(entries)))
 
+    def set_batch_graphs(self, on):
+        self._check(lib().hs_set_batch_graphs(self._h, 1 if on else 0))
+
+    def batch_stats(self):
+        a = C.c_uint64()
+        b = C.c_uint64()
+        self._check(lib().hs_batch_stats(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def batch(self, grooms, max_colliders=1):
+        return Batch(self, grooms, max_colliders)
+
     def profile_breakdown(self):
         a = C.c_double()
         b = C.c_double()
@@ -284,6 +341,52 @@ class Context:
         return Groom(self, strands_aos, num_points, strand_length=strand_length, seg_length=seg_length,
                      wind_noise=wind_noise, noise_dir=noise_dir, grid_dim=grid_dim,
                      grid_origin=grid_origin, grid_h=grid_h, keep_corrections=keep_corrections)
+
+
+class Batch:
+    """A fixed set of grooms stepped together by ``hs_step_batch``.
+
+    The item, collider and parameter arrays are allocated once; numpy views
+    (``time``, ``colliders``, ``wind``, ``num_colliders``, ``params``) update
+    them in place between steps, so a step costs one C call.
+    """
+
+    def __init__(self, ctx, grooms, max_colliders=1):
+        n = len(grooms)
+        self.ctx = ctx
+        self.grooms = list(grooms)
+        self.n = n
+        self.max_colliders = max_colliders
+        self._items = (HsBatchItem * n)()
+        self._cols = (HsCollider * (n * max(1, max_colliders)))()
+        self._params = (HsStepParams * n)()
+        base = C.addressof(self._cols)
+        for k, g in enumerate(grooms):
+            self._params[k] = step_params()
+            it = self._items[k]
+            it.groom_id = g.id
+            it.colliders = C.cast(base + k * max(1, max_colliders) * C.sizeof(HsCollider), C.POINTER(HsCollider))
+            it.num_colliders = 0
+            it.params = C.pointer(self._params[k])
+        self.items = np.ctypeslib.as_array(self._items)
+        self.params = np.ctypeslib.as_array(self._params)
+        self.colliders = np.frombuffer(self._cols, dtype=np.float32).reshape(n, max(1, max_colliders), 48)
+
+    def set_time(self, dt, total_time):
+        self.items["time"]["deltaTime"] = dt
+        self.items["time"]["totalTime"] = total_time
+
+    def set_colliders(self, cols, num=None):
+        """cols: (n, m, 48) float32; every groom gets m colliders (params.num_colliders too)."""
+        cols = np.asarray(cols, dtype=np.float32)
+        m = cols.shape[1]
+        assert m <= self.max_colliders
+        self.colliders[:, :m] = cols
+        self.items["num_colliders"] = m
+        self.params["num_colliders"] = m if num is None else num
+
+    def step(self):
+        self.ctx._check(lib().hs_step_batch(self.ctx._h, self._items, self.n))
 
 
 class Groom:
@@ -342,7 +445,115 @@ class Groom:
                                            out.ctypes.data_as(C.POINTER(HsGridCell))))
         return out
 
+    def grid_near_limit(self):
+        """(cell, step) pairs so far with a GridCell component within 2x of the int32 limit."""
+        v = C.c_uint64(0)
+        self.ctx._check(lib().hs_grid_near_limit(self.ctx._h, self.id, C.byref(v)))
+        return v.value
+
     def destroy(self):
         if self.id:
             self.ctx._check(lib().hs_groom_destroy(self.ctx._h, self.id))
             self.id = 0
+
+
+# --------------------------------------------------------------------------
+# Served jobs
+# --------------------------------------------------------------------------
+def keyframes(keys):
+    """(K, 7) array-like of [t, wind xyz, sphere xyz] -> ctypes HsKeyframe array."""
+    k = np.ascontiguousarray(keys, dtype=np.float32).reshape(-1, 7)
+    arr = (HsKeyframe * len(k))()
+    np.frombuffer(arr, dtype=np.float32).reshape(-1, 7)[:] = k
+    return arr
+
+
+def keyframes_eval(keys, t):
+    arr = keys if isinstance(keys, C.Array) else keyframes(keys)
+    w = (C.c_float * 3)()
+    s = (C.c_float * 3)()
+    lib().hs_keyframes_eval(arr, len(arr), C.c_float(t), w, s)
+    return tuple(w), tuple(s)
+
+
+class Server:
+    """Continuous-batching job server (hs_server_*): submit jobs, wait for their results."""
+
+    def __init__(self, device=0, hbm_budget=1 << 30, max_points_per_step=1 << 24, use_graphs=False):
+        cfg = HsServerConfig(device, hbm_budget, max_points_per_step, 1 if use_graphs else 0)
+        self._h = C.c_void_p()
+        rc = lib().hs_server_create(C.byref(cfg), C.byref(self._h))
+        if rc != HS_OK:
+            raise HairSimError(rc, "hs_server_create failed")
+        self._keep = {}
+
+    def _check(self, rc, what):
+        if rc != HS_OK:
+            msg = C.c_char_p()
+            lib().hs_server_error(self._h, C.byref(msg))
+            raise HairSimError(rc, f"{what}: {(msg.value or b'').decode()}")
+
+    def submit(self, strands_aos, num_points, steps, dt, keys, params, *, strand_length=2.5, seg_length=None,
+               wind_noise=None, noise_dir=(1.0, 0.0, 0.0), grid_dim=0, grid_origin=(0.0, 0.0, 0.0), grid_h=1.0):
+        """Queue one job; returns (job_id, result array filled when the job is done)."""
+        aos = np.ascontiguousarray(strands_aos, dtype=np.float32).reshape(-1)
+        S = aos.size // (12 * num_points)
+        desc = HsGroomDesc()
+        desc.num_strands = S
+        desc.num_points = num_points
+        desc.strand_length = strand_length
+        seg = None if seg_length is None else np.ascontiguousarray(seg_length, dtype=np.float32)
+        noise = None if wind_noise is None else np.ascontiguousarray(wind_noise, dtype=np.float32)
+        desc.seg_length = _fptr(seg) if seg is not None else None
+        desc.wind_noise = _fptr(noise) if noise is not None else None
+        desc.noise_dir = (C.c_float * 3)(*noise_dir)
+        desc.grid_dim = grid_dim
+        desc.grid_origin = (C.c_float * 3)(*grid_origin)
+        desc.grid_h = grid_h
+        desc.flags = 0
+        karr = keys if isinstance(keys, C.Array) else keyframes(keys)
+        result = np.empty_like(aos)
+        job = HsJob()
+        job.groom = desc
+        job.strands_aos = _fptr(aos)
+        job.steps = steps
+        job.dt = dt
+        job.keys = C.cast(karr, C.POINTER(HsKeyframe))
+        job.num_keys = len(karr)
+        job.params = params
+        job.result_aos = _fptr(result)
+        jid = C.c_uint64()
+        self._check(lib().hs_server_submit(self._h, C.byref(job), C.byref(jid)), "hs_server_submit")
+        self._keep[jid.value] = (aos, seg, noise, karr, result)
+        return jid.value, result
+
+    def wait(self, job_id, timeout=-1.0):
+        rc = lib().hs_server_wait(self._h, job_id, float(timeout))
+        if rc == HS_OK:
+            return self._keep.pop(job_id)[4]
+        self._check(rc, f"hs_server_wait({job_id})")
+
+    def cancel_pending(self):
+        n = C.c_uint64()
+        self._check(lib().hs_server_cancel_pending(self._h, C.byref(n)), "hs_server_cancel_pending")
+        return n.value
+
+    def drain(self):
+        self._check(lib().hs_server_drain(self._h), "hs_server_drain")
+
+    def stats(self):
+        s = HsServerStats()
+        self._check(lib().hs_server_get_stats(self._h, C.byref(s)), "hs_server_get_stats")
+        return {name: getattr(s, name) for name, _ in HsServerStats._fields_}
+
+    def close(self):
+        if self._h:
+            lib().hs_server_destroy(self._h)
+            self._h = C.c_void_p()
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

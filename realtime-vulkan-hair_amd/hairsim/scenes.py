@@ -181,6 +181,12 @@ def sphere_collider(center):
     return m
 
 
+def strand_bytes(S, N):
+    """The strand-state part of a step's bytes: 48 per non-root point (pos+vel
+    read and written), 16 per strand (root + rest length)."""
+    return S * (48 * (N - 1) + 16)
+
+
 def algorithmic_bytes(S, N, G, hair_hair):
     """Bytes one step must move: 48 per non-root point (pos+vel read and
     written), 16 per strand (root + rest length), 32 per voxel (clear + one

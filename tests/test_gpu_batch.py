@@ -62,3 +62,84 @@ def test_batch_rejects_duplicates(ctx):
         ctx.step_batch([(gr, wl.dt, 0.0, [scenes.sphere_collider(wl.sphere(0, 0))], p)] * 2)
     assert e.value.code == hairsim.HS_ERR_INVALID
     gr.destroy()
+
+
+def _run_batch_object(ctx, wl, grs, steps, graphs, reads_between=False):
+    ctx.set_batch_graphs(graphs)
+    b = ctx.batch(grs, max_colliders=1)
+    snaps = []
+    for k in range(steps):
+        b.set_time(wl.dt, float(wl.time(k)))
+        b.set_colliders(np.stack([scenes.sphere_collider(wl.sphere(k, gi)) for gi in range(len(grs))])[:, None])
+        b.params["wind"] = np.stack([wl.wind(k, gi) for gi in range(len(grs))])
+        b.params["damping"] = wl.damping
+        b.params["sphere_radius"] = wl.sphere_radius
+        b.params["friction"] = wl.friction
+        b.params["hair_hair"] = 1
+        b.step()
+        if reads_between:
+            # eager work and a synchronisation between graph replays
+            snaps.append(grs[k % len(grs)].read()[0])
+    ctx.set_batch_graphs(False)
+    return snaps
+
+
+def test_batch_graph_replays_equal_eager(ctx):
+    """Captured-graph batch steps: every replay gives the eager result bit for bit.
+
+    Replays alternate between the two pinned argument slots, with reads (device
+    to host copies + synchronisation) between them.
+    """
+    wl = small_batch(n=9, seed=21)
+    steps = 7
+    eager = [make_gpu_groom(ctx, wl, gi) for gi in range(len(wl.grooms))]
+    snaps_e = _run_batch_object(ctx, wl, eager, steps, graphs=False, reads_between=True)
+    calls0, caps0 = ctx.batch_stats()
+    graph = [make_gpu_groom(ctx, wl, gi) for gi in range(len(wl.grooms))]
+    snaps_g = _run_batch_object(ctx, wl, graph, steps, graphs=True, reads_between=True)
+    calls1, caps1 = ctx.batch_stats()
+    assert calls1 - calls0 == steps
+    # shapes: first step (no pending friction) and the rest; one graph per pinned slot
+    assert caps1 - caps0 == 3
+    for k, (a, b) in enumerate(zip(snaps_e, snaps_g)):
+        assert_bitwise(b, a, f"replay {k}", wl.grooms[k % len(wl.grooms)].N)
+    for gi in range(len(wl.grooms)):
+        assert_bitwise(graph[gi].read()[0], eager[gi].read()[0], f"groom {gi}", wl.grooms[gi].N)
+        assert np.array_equal(graph[gi].read_grid(), eager[gi].read_grid()), gi
+    # the batch object path equals the list path / stepping alone
+    for gi in range(len(wl.grooms)):
+        gr = make_gpu_groom(ctx, wl, gi)
+        for k in range(steps):
+            gr.step(wl.dt, float(wl.time(k)), [scenes.sphere_collider(wl.sphere(k, gi))], gpu_params(wl, k, gi))
+        assert_bitwise(eager[gi].read()[0], gr.read()[0], f"alone {gi}", wl.grooms[gi].N)
+        gr.destroy()
+    for gr in eager + graph:
+        gr.destroy()
+
+
+def test_batch_ellipsoids_in_blob(ctx):
+    """Grooms with ellipsoid colliders in a batch read their own colliders from the argument blob."""
+    wl = small_batch(n=4, seed=33)
+    demo = [scenes.sphere_collider((0.0, 0.0, 1.2))]
+    for j, t in enumerate([(0.3, 0.8, 0.0), (-0.5, 0.2, 0.4), (0.0, -0.3, -0.6)]):
+        demo.append(hairsim.collider_from_trs(t, (10.0 * j, 20.0, 5.0), (0.6, 0.9, 0.7)).as_array())
+    def cols_for(gi, k):
+        return [scenes.sphere_collider(wl.sphere(k, gi))] + demo[1:1 + (gi % 4)]
+    alone = []
+    for gi in range(len(wl.grooms)):
+        gr = make_gpu_groom(ctx, wl, gi)
+        for k in range(4):
+            cl = cols_for(gi, k)
+            gr.step(wl.dt, float(wl.time(k)), cl, gpu_params(wl, k, gi, ncol=len(cl)))
+        alone.append(gr.read()[0])
+        gr.destroy()
+    for graphs in (False, True):
+        ctx.set_batch_graphs(graphs)
+        grs = [make_gpu_groom(ctx, wl, gi) for gi in range(len(wl.grooms))]
+        for k in range(4):
+            ctx.step_batch([(gr, wl.dt, float(wl.time(k)), cols_for(gi, k),
+                             gpu_params(wl, k, gi, ncol=len(cols_for(gi, k)))) for gi, gr in enumerate(grs)])
+        for gi, gr in enumerate(grs):
+            assert_bitwise(gr.read()[0], alone[gi], f"groom {gi} graphs={graphs}", wl.grooms[gi].N)
+            gr.destroy()
+    ctx.set_batch_graphs(False)

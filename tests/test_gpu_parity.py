@@ -175,3 +175,51 @@ def test_errors_are_status_codes(ctx):
     with pytest.raises(hairsim.HairSimError) as e:
         gpu.read()
     assert e.value.code == hairsim.HS_ERR_NOT_FOUND
+
+
+def _stacked_strands(S, root, drop):
+    """S identical two-point strands: root, and a point `drop` straight below it, at rest."""
+    aos = np.zeros((S, 3, 2, 4), dtype=np.float32)
+    aos[:, 0, 0, :3] = root
+    aos[:, 0, 1, :3] = (root[0], root[1] - drop, root[2])
+    aos[:, 0, :, 3] = 1.0
+    return aos.reshape(-1)
+
+
+@pytest.mark.parametrize("S", [1100, 76800])
+def test_stacked_voxel_near_limit_and_list_overflow(ctx, S):
+    """Every strand's moving point sits on one voxel corner (weight 1 in one cell).
+
+    S = 1100: density 1.1e9 >= 2^30 in one cell -> one near-limit cell per step.
+    S = 76800: 600 workgroup boxes overlap one brick, more than a brick's box
+    list holds, so the reduce falls back to scanning every box; the density
+    wraps in int32 exactly as the shader's atomicAdd does.
+    """
+    G, h, origin = 16, 0.25, (-2.0, -2.0, -2.0)
+    root, drop = (0.0, 0.5, 0.0), 0.5
+    aos = _stacked_strands(S, root, drop)
+    seg = np.full(S, drop, dtype=np.float32)
+    gpu = ctx.groom(aos, 2, seg_length=seg, grid_dim=G, grid_origin=origin, grid_h=h)
+    ref = aos.copy()
+    grid = None
+    try:
+        for k in range(3):
+            p = hairsim.step_params(damping=0.9, num_colliders=0, friction=0.05)
+            gpu.step(1.0 / 64, (k + 1) / 64, [], p)
+            op = oracle.Params(dt=1.0 / 64, damping=0.9, friction=0.05, hair_hair=True, grid_dim=G,
+                               grid_origin=origin, grid_h=h, seg_length=seg)
+            grid = oracle.step32(op, ref, 2, grid)
+            out, _ = gpu.read()
+            o = out.reshape(-1, 3, 2, 4).copy()
+            r = ref.reshape(-1, 3, 2, 4).copy()
+            o[:, 2] = 0
+            r[:, 2] = 0
+            assert_bitwise(o, r, f"strands after step {k}", 2)
+            gg = gpu.read_grid()
+            assert np.array_equal(gg, grid.reshape(-1)), f"grid after step {k}"
+            near = int((np.abs(gg.astype(np.int64)) >= 2 ** 30).reshape(-1, 4).any(axis=1).sum())
+            if S == 1100:
+                assert near == 1
+            assert gpu.grid_near_limit() == (k + 1) * near
+    finally:
+        gpu.destroy()

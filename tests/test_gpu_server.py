@@ -1,0 +1,118 @@
+"""Served jobs (hs_server_*): continuous batching over the batched step.
+
+Jobs of different shapes and lengths arrive while others run; the admission
+budget forces some to wait.  Each job's final state must equal stepping that
+job alone with hs_step on the same keyframed inputs, bit for bit (the batched
+kernels compute every groom exactly as the single-groom kernels do).
+"""
+import time
+
+import numpy as np
+import pytest
+
+import hairsim
+from hairsim import scenes
+from helpers import assert_bitwise
+
+pytestmark = pytest.mark.gpu
+
+
+def _job_set(seed, n):
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for j in range(n):
+        S = int(rng.integers(200, 2500))
+        N = int(rng.choice([8, 16, 32]))
+        roots, nrm = scenes.hemisphere_roots(rng, S)
+        L = rng.uniform(0.8, 1.2, S)
+        aos, seg = scenes.straight_strands(roots, nrm, N, L)
+        steps = int(rng.integers(4, 14))
+        keys = np.zeros((3, 7), dtype=np.float32)
+        keys[:, 0] = [0.0, 0.1, 0.25]
+        keys[:, 1:4] = rng.normal(0.0, 1.5, (3, 3))
+        keys[:, 4:7] = rng.uniform(-1.2, 1.2, (3, 3))
+        noise = rng.normal(0.0, 0.1, S).astype(np.float32) if j % 2 else None
+        jobs.append(dict(aos=aos, N=N, seg=seg, steps=steps, keys=keys, noise=noise))
+    return jobs
+
+
+def _params(ncol=1):
+    return hairsim.step_params(damping=0.9, sphere_radius=0.3, friction=0.05, num_colliders=ncol, hair_hair=True)
+
+
+GRID = dict(grid_dim=16, grid_origin=(-2.2, -2.2, -2.2), grid_h=4.4 / 16)
+DT = 1.0 / 60.0
+
+
+def _alone(ctx, job):
+    gr = ctx.groom(job["aos"], job["N"], seg_length=job["seg"], wind_noise=job["noise"], **GRID)
+    karr = hairsim.keyframes(job["keys"])
+    for k in range(job["steps"]):
+        t = float(np.float32(k + 1) * np.float32(DT))
+        wind, sph = hairsim.keyframes_eval(karr, t)
+        p = hairsim.step_params(wind=wind, damping=0.9, sphere_radius=0.3, friction=0.05, num_colliders=1,
+                                hair_hair=True)
+        gr.step(DT, t, [scenes.sphere_collider(sph)], p)
+    out, _ = gr.read()
+    gr.destroy()
+    return out
+
+
+def test_served_jobs_equal_alone(ctx):
+    jobs = _job_set(77, 10)
+    refs = [_alone(ctx, j) for j in jobs]
+    sizes = [hairsim.groom_bytes(j["aos"].size // (12 * j["N"]), j["N"], 16) + j["aos"].nbytes for j in jobs]
+    budget = int(sum(sorted(sizes)[-3:]) * 1.05)  # about three jobs resident at once
+    with hairsim.Server(0, hbm_budget=budget, max_points_per_step=60000) as srv:
+        ids = []
+        for i, j in enumerate(jobs):
+            jid, _ = srv.submit(j["aos"], j["N"], j["steps"], DT, j["keys"], _params(), seg_length=j["seg"],
+                                wind_noise=j["noise"], **GRID)
+            ids.append(jid)
+            if i % 3 == 2:
+                time.sleep(0.01)  # later jobs arrive while earlier ones run
+        for i, jid in enumerate(ids):
+            out = srv.wait(jid, timeout=60.0)
+            assert_bitwise(out, refs[i], f"job {i}", jobs[i]["N"])
+        srv.drain()
+        st = srv.stats()
+    assert st["completed"] == len(jobs)
+    assert st["peak_resident_bytes"] <= budget
+    assert st["launches"] >= max(j["steps"] for j in jobs)
+    # more than one job per launch happened (batching), and not all at once (budget)
+    assert 1 < st["peak_jobs_per_launch"] < len(jobs)
+    assert st["job_steps"] == sum(j["steps"] for j in jobs)
+    assert st["p99_step_ms"] >= st["p50_step_ms"] > 0.0
+
+
+def test_server_rejects_and_cancels(ctx):
+    jobs = _job_set(5, 4)
+    j = jobs[0]
+    small = hairsim.groom_bytes(j["aos"].size // (12 * j["N"]), j["N"], 16)
+    with hairsim.Server(0, hbm_budget=small // 2, max_points_per_step=1 << 24) as srv:
+        with pytest.raises(hairsim.HairSimError) as e:
+            srv.submit(j["aos"], j["N"], 3, DT, j["keys"], _params(), seg_length=j["seg"], **GRID)
+        assert e.value.code == hairsim.HS_ERR_BUDGET
+    with hairsim.Server(0, hbm_budget=1 << 30, max_points_per_step=1) as srv:
+        with pytest.raises(hairsim.HairSimError) as e:
+            srv.submit(j["aos"], j["N"], 3, DT, j["keys"], _params(), seg_length=j["seg"], **GRID)
+        assert e.value.code == hairsim.HS_ERR_BUDGET
+    # a one-job-at-a-time server: queued jobs can be cancelled; the admitted one finishes
+    pts = max(x["aos"].size // 12 for x in jobs)
+    with hairsim.Server(0, hbm_budget=1 << 30, max_points_per_step=pts) as srv:
+        ids = [srv.submit(x["aos"], x["N"], 40, DT, x["keys"], _params(), seg_length=x["seg"], **GRID)[0]
+               for x in jobs]
+        n = srv.cancel_pending()
+        srv.drain()
+        st = srv.stats()
+        assert st["completed"] + n == len(jobs)
+        assert st["cancelled"] == n
+        done = cancelled = 0
+        for i in ids:
+            try:
+                srv.wait(i, 0.0)
+                done += 1
+            except hairsim.HairSimError as e:
+                assert e.code == hairsim.HS_ERR_NOT_FOUND
+                cancelled += 1
+        assert (done, cancelled) == (st["completed"], n)

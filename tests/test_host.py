@@ -137,3 +137,16 @@ def test_bench_rejects_more_than_one_gpu():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_keyframes_eval_piecewise_linear():
+    """hs_keyframes_eval: linear between keys, held before the first and after the last."""
+    keys = np.array([[0.0, 1, 2, 3, 10, 20, 30],
+                     [0.5, 3, 2, 1, 20, 20, 40],
+                     [1.0, 3, 2, 1, 20, 20, 40]], dtype=np.float32)
+    w, s = hairsim.keyframes_eval(keys, 0.25)
+    assert w == (2.0, 2.0, 2.0) and s == (15.0, 20.0, 35.0)
+    assert hairsim.keyframes_eval(keys, -1.0) == ((1.0, 2.0, 3.0), (10.0, 20.0, 30.0))
+    assert hairsim.keyframes_eval(keys, 7.0) == ((3.0, 2.0, 1.0), (20.0, 20.0, 40.0))
+    assert hairsim.keyframes_eval(keys, 0.5) == ((3.0, 2.0, 1.0), (20.0, 20.0, 40.0))
+    assert hairsim.keyframes_eval(keys[:1], 0.3) == ((1.0, 2.0, 3.0), (10.0, 20.0, 30.0))
