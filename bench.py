@@ -39,9 +39,9 @@ import numpy as np  # noqa: E402
 PEAK_HBM_GBS = 8000.0
 
 
-def make_params(hairsim, wl, k):
+def make_params(hairsim, wl, k, mesh_id=0):
     return hairsim.step_params(wind=wl.wind(k), damping=wl.damping, sphere_radius=wl.sphere_radius,
-                               friction=wl.friction, num_colliders=1, hair_hair=wl.hair_hair)
+                               friction=wl.friction, num_colliders=1, hair_hair=wl.hair_hair, mesh_id=mesh_id)
 
 
 def cpu_baseline(wl, budget_s=12.0):
@@ -51,19 +51,22 @@ def cpu_baseline(wl, budget_s=12.0):
     from hairsim import scenes
 
     g = wl.grooms[0]
-    S = min(g.S, 20000)
+    # the oracle scans every mesh triangle per point: a smaller sample then
+    S = min(g.S, 2000 if wl.mesh is not None else 20000)
     strands = g.strands[:S].copy()
     seg = g.seg[:S].copy()
     noise = None if g.noise is None else g.noise[:S].copy()
     threads = min(os.cpu_count() or 1, 16)
     grid = None
+    recs = None if wl.mesh is None else oracle.mesh_prepare(*wl.mesh)
     steps = 0
     t0 = time.perf_counter()
     while True:
         p = oracle.Params(dt=wl.dt, wind=wl.wind(steps), damping=wl.damping, sphere_radius=wl.sphere_radius,
                           friction=wl.friction, hair_hair=wl.hair_hair, grid_dim=g.grid_dim,
                           grid_origin=g.grid_origin, grid_h=g.grid_h,
-                          colliders=scenes.sphere_collider(wl.sphere(steps)), seg_length=seg, wind_noise=noise)
+                          colliders=scenes.sphere_collider(wl.sphere(steps)), seg_length=seg, wind_noise=noise,
+                          mesh=recs, mesh_margin=wl.mesh_margin)
         grid = oracle.step32_mt(p, strands, g.N, grid, threads)
         steps += 1
         el = time.perf_counter() - t0
@@ -71,7 +74,8 @@ def cpu_baseline(wl, budget_s=12.0):
             break
     return {"value": S * g.N * steps / el, "unit": "point-updates/s", "cores": threads, "kind": "port",
             "sample": f"{S} of {g.S} strands x {g.N} points, {g.grid_dim}^3 grid, {steps} steps in {el:.1f} s "
-                      f"(FP32 C restatement of compute.comp, OpenMP)"}
+                      f"(FP32 C restatement of compute.comp, OpenMP"
+                      f"{'; mesh contact by scanning every triangle' if recs is not None else ''})"}
 
 
 def copy_probe(mib, reps=5):
@@ -102,11 +106,14 @@ def bench_single(args, hairsim, scenes, wl):
     ctx = hairsim.Context(0)
     groom = ctx.groom(g.strands, g.N, seg_length=g.seg, wind_noise=g.noise, noise_dir=g.noise_dir,
                       grid_dim=g.grid_dim, grid_origin=g.grid_origin, grid_h=g.grid_h)
+    mesh_id = 0
+    if wl.mesh is not None:
+        mesh_id = ctx.mesh(*wl.mesh, grid_dim=wl.mesh_grid, margin=wl.mesh_margin)
     # per-step inputs prepared ahead so the timed loop is launches only
     total = args.warmup + 2 * args.steps
     cols = [[hairsim.HsCollider.from_buffer_copy(scenes.sphere_collider(wl.sphere(k % wl.steps)).tobytes())]
             for k in range(total)]
-    prm = [make_params(hairsim, wl, k % wl.steps) for k in range(total)]
+    prm = [make_params(hairsim, wl, k % wl.steps, mesh_id) for k in range(total)]
     tms = [float(wl.time(k % wl.steps)) for k in range(total)]
 
     k = 0
@@ -162,7 +169,8 @@ def bench_single(args, hairsim, scenes, wl):
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"{wl.name}: {g.S} strands x {g.N} points, grid {g.grid_dim}^3, "
-                               f"wind + noise, orbiting sphere, friction {wl.friction}",
+                               f"wind + noise, orbiting sphere, friction {wl.friction}"
+                               + (f", head mesh {len(wl.mesh[1])} triangles" if wl.mesh is not None else ""),
                    "strands": g.S, "points": g.N, "grid_dim": g.grid_dim, "hair_hair": wl.hair_hair},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": load_traffic(wl.name),

@@ -233,6 +233,30 @@ int hs_read_grid(hs_ctx* ctx, uint32_t groom_id, hs_grid_cell* cells);
 int hs_grid_near_limit(hs_ctx* ctx, uint32_t groom_id, uint64_t* cells);
 
 /* ------------------------------------------------------------------------ */
+/* Triangle-mesh collider (no reference counterpart: compute.comp:155-184     */
+/* handles a sphere and ellipsoids only).  Rule, per start-of-step point x:   */
+/* q = the mesh point nearest to x (lowest triangle id on ties); contact when */
+/* |x - q| < margin and x is inside (dot(x - q, n) < 0, n the angle-weighted  */
+/* pseudo-normal of q's face/edge/vertex); the penalty k |x - q|              */
+/* normalize(q - x) then counts as one more collider in the average of        */
+/* compute.comp:182-184, after the UBO colliders.  Points deeper inside than  */
+/* `margin` feel no mesh force.  A step uses the mesh hs_step_params.mesh_id  */
+/* names (0: none).                                                           */
+/* ------------------------------------------------------------------------ */
+
+typedef struct hs_mesh_desc {
+    uint32_t grid_dim[3];  /* triangle-grid cells per axis over the mesh bounds grown by margin */
+    float margin;          /* contact range, > 0 */
+} hs_mesh_desc;
+
+/* vertices: num_vertices x 3 floats; triangles: num_triangles x 3 indices,
+ * counter-clockwise seen from outside; the mesh must be closed (every edge
+ * shared by exactly two triangles).  Copies everything. */
+int hs_mesh_create(hs_ctx* ctx, const float* vertices, uint32_t num_vertices, const uint32_t* triangles,
+                   uint32_t num_triangles, const hs_mesh_desc* desc, uint32_t* mesh_id);
+int hs_mesh_destroy(hs_ctx* ctx, uint32_t mesh_id);
+
+/* ------------------------------------------------------------------------ */
 /* Kernel timing (no reference counterpart: the reference has no timers)     */
 /* ------------------------------------------------------------------------ */
 

@@ -8,6 +8,7 @@
 
 #include <math.h>
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
@@ -62,4 +63,103 @@ void ho_step_f64(const ho_params* p, double* strands, int S, int N, int32_t* gri
 void ho_step_f32_mt(const ho_params* p, float* strands, int S, int N, int32_t* grid, int threads)
 {
     step_impl_f32(p, strands, S, N, grid, threads);
+}
+
+/* ---- mesh preparation (binary64 normals, rounded to float) ---- */
+typedef struct { double x, y, z; } d3;
+static d3 d3_sub(d3 a, d3 b) { d3 r = {a.x - b.x, a.y - b.y, a.z - b.z}; return r; }
+static d3 d3_add(d3 a, d3 b) { d3 r = {a.x + b.x, a.y + b.y, a.z + b.z}; return r; }
+static d3 d3_mul(d3 a, double s) { d3 r = {a.x * s, a.y * s, a.z * s}; return r; }
+static double d3_dot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static d3 d3_cross(d3 a, d3 b)
+{
+    d3 r = {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+    return r;
+}
+static d3 d3_unit(d3 a)
+{
+    double l = sqrt(d3_dot(a, a));
+    if (l > 0) return d3_mul(a, 1.0 / l);
+    d3 z = {0, 0, 0};
+    return z;
+}
+
+typedef struct { uint32_t lo, hi; int face; } edge_rec;
+static int edge_cmp(const void* pa, const void* pb)
+{
+    const edge_rec* a = (const edge_rec*)pa;
+    const edge_rec* b = (const edge_rec*)pb;
+    if (a->lo != b->lo) return a->lo < b->lo ? -1 : 1;
+    if (a->hi != b->hi) return a->hi < b->hi ? -1 : 1;
+    return (a->face > b->face) - (a->face < b->face);
+}
+
+int ho_mesh_prepare(const float* v, int nv, const uint32_t* t, int nt, float* out)
+{
+    if (nv < 4 || nt < 4) return -1;
+    for (long k = 0; k < 3L * nt; ++k)
+        if (t[k] >= (uint32_t)nv) return -1;
+    d3* fn = (d3*)malloc(sizeof(d3) * (size_t)nt);
+    d3* vn = (d3*)calloc((size_t)nv, sizeof(d3));
+    edge_rec* e = (edge_rec*)malloc(sizeof(edge_rec) * 3 * (size_t)nt);
+    int* other = (int*)malloc(sizeof(int) * 3 * (size_t)nt); /* [face][k]: the face across edge k */
+    int rc = 0;
+    for (int f = 0; f < nt; ++f) {
+        d3 p[3];
+        for (int k = 0; k < 3; ++k) {
+            const float* q = v + 3 * t[3 * f + k];
+            d3 w = {q[0], q[1], q[2]};
+            p[k] = w;
+        }
+        fn[f] = d3_unit(d3_cross(d3_sub(p[1], p[0]), d3_sub(p[2], p[0])));
+        for (int k = 0; k < 3; ++k) {
+            d3 e1 = d3_unit(d3_sub(p[(k + 1) % 3], p[k]));
+            d3 e2 = d3_unit(d3_sub(p[(k + 2) % 3], p[k]));
+            double c = d3_dot(e1, e2);
+            double ang = acos(c < -1.0 ? -1.0 : (c > 1.0 ? 1.0 : c));
+            uint32_t vi = t[3 * f + k];
+            vn[vi] = d3_add(vn[vi], d3_mul(fn[f], ang));
+            uint32_t a = t[3 * f + k], b = t[3 * f + (k + 1) % 3];
+            edge_rec r = {a < b ? a : b, a < b ? b : a, 3 * f + k};
+            e[3 * f + k] = r;
+        }
+    }
+    qsort(e, 3 * (size_t)nt, sizeof(edge_rec), edge_cmp);
+    for (long i = 0; i < 3L * nt && rc == 0; i += 2) {
+        if (i + 1 >= 3L * nt || e[i].lo != e[i + 1].lo || e[i].hi != e[i + 1].hi) { rc = -1; break; }
+        if (i + 2 < 3L * nt && e[i + 2].lo == e[i].lo && e[i + 2].hi == e[i].hi) { rc = -1; break; }
+        other[e[i].face] = e[i + 1].face / 3;
+        other[e[i + 1].face] = e[i].face / 3;
+    }
+    for (int f = 0; f < nt && rc == 0; ++f) {
+        float* r = out + (size_t)f * HO_MESH_TRI_FLOATS;
+        memset(r, 0, sizeof(float) * HO_MESH_TRI_FLOATS);
+        for (int k = 0; k < 3; ++k) {
+            const float* q = v + 3 * t[3 * f + k];
+            r[4 * k] = q[0];
+            r[4 * k + 1] = q[1];
+            r[4 * k + 2] = q[2];
+        }
+        r[12] = (float)fn[f].x;
+        r[13] = (float)fn[f].y;
+        r[14] = (float)fn[f].z;
+        for (int k = 0; k < 3; ++k) {
+            int g = other[3 * f + k];
+            /* the two adjacent faces, lower index first */
+            int f0 = f < g ? f : g, f1 = f < g ? g : f;
+            d3 en = d3_unit(d3_add(fn[f0], fn[f1]));
+            r[16 + 4 * k] = (float)en.x;
+            r[16 + 4 * k + 1] = (float)en.y;
+            r[16 + 4 * k + 2] = (float)en.z;
+            d3 w = d3_unit(vn[t[3 * f + k]]);
+            r[28 + 4 * k] = (float)w.x;
+            r[28 + 4 * k + 1] = (float)w.y;
+            r[28 + 4 * k + 2] = (float)w.z;
+        }
+    }
+    free(fn);
+    free(vn);
+    free(e);
+    free(other);
+    return rc;
 }

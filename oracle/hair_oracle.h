@@ -60,7 +60,29 @@ typedef struct ho_params {
     const float* seg_length;  /* S floats: segment rest length per strand   */
     const float* wind_noise;  /* S floats or NULL                           */
     float noise_dir[3];
+    /* Triangle-mesh collider (no reference counterpart; see ho_mesh_prepare):
+     * mesh_ntris records of HO_MESH_TRI_FLOATS floats, or 0 for none. */
+    const float* mesh_tris;
+    int mesh_ntris;
+    float mesh_margin;
 } ho_params;
+
+/* Mesh collider.  The reference has no mesh collider (compute.comp:155-184
+ * handles a sphere and ellipsoids); the rule restated here is this build's:
+ *   q = the point of the mesh nearest to x, over every triangle (closest point
+ *       on a triangle: Ericson, Real-Time Collision Detection, 2005, 5.1.5),
+ *       the lowest triangle index winning ties;
+ *   contact when |x - q|^2 < margin^2 and x is inside: dot(x - q, n) < 0 with
+ *       n the angle-weighted pseudo-normal of the feature q lies on (face,
+ *       edge or vertex; Baerentzen & Aanaes, IEEE TVCG 11(3), 2005);
+ *   then, as one more collider after the UBO ones: acc += (k*d) * normalize(q - x),
+ *       d = |x - q|, hits += 1.
+ * ho_mesh_prepare writes, per triangle, 10 groups of 4 floats (w = 0):
+ * a, b, c, face normal, edge normals ab bc ca, vertex normals a b c; normals
+ * computed in binary64 and rounded.  Returns 0, or -1 for an index out of
+ * range or an edge not shared by exactly two triangles. */
+#define HO_MESH_TRI_FLOATS 40
+int ho_mesh_prepare(const float* verts, int nv, const uint32_t* tris, int nt, float* out);
 
 /* One step over S strands of N points, in place on `strands` (reference AoS
  * layout, 12*N values per strand).  When hair_hair is set, `grid` (G^3 cells

@@ -123,6 +123,89 @@ static V3 F(gather)(const ho_params* p, V3 pos, const int32_t* grid)
     return gv;
 }
 
+/* Closest point on triangle abc to x (Ericson 2005, 5.1.5) and the feature it
+ * lies on: 0..2 vertex a/b/c, 3 edge ab, 4 edge bc, 5 edge ca, 6 face. */
+static V3 F(closest_on_tri)(V3 x, V3 a, V3 b, V3 c, int* feature)
+{
+    V3 ab = F(vsub)(b, a), ac = F(vsub)(c, a), ap = F(vsub)(x, a);
+    R d1 = F(dot)(ab, ap), d2 = F(dot)(ac, ap);
+    if (d1 <= 0 && d2 <= 0) { *feature = 0; return a; }
+    V3 bp = F(vsub)(x, b);
+    R d3 = F(dot)(ab, bp), d4 = F(dot)(ac, bp);
+    if (d3 >= 0 && d4 <= d3) { *feature = 1; return b; }
+    R vc = d1 * d4 - d3 * d2;
+    if (vc <= 0 && d1 >= 0 && d3 <= 0) {
+        R v = d1 / (d1 - d3);
+        *feature = 3;
+        return F(vadd)(a, F(smul)(v, ab));
+    }
+    V3 cp = F(vsub)(x, c);
+    R d5 = F(dot)(ab, cp), d6 = F(dot)(ac, cp);
+    if (d6 >= 0 && d5 <= d6) { *feature = 2; return c; }
+    R vb = d5 * d2 - d1 * d6;
+    if (vb <= 0 && d2 >= 0 && d6 <= 0) {
+        R w = d2 / (d2 - d6);
+        *feature = 5;
+        return F(vadd)(a, F(smul)(w, ac));
+    }
+    R va = d3 * d6 - d5 * d4;
+    if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+        R w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        *feature = 4;
+        return F(vadd)(b, F(smul)(w, F(vsub)(c, b)));
+    }
+    R denom = (R)1 / ((va + vb) + vc);
+    R v = vb * denom, w = vc * denom;
+    *feature = 6;
+    return F(vadd)(F(vadd)(a, F(smul)(v, ab)), F(smul)(w, ac));
+}
+
+/* Mesh contact of the rule in hair_oracle.h, by brute force over every
+ * triangle; adds the penalty to *acc and returns 1 on contact. */
+static int F(mesh_contact)(const ho_params* p, V3 x, R k, V3* acc)
+{
+    static const int slot_of_feature[7] = {7, 8, 9, 4, 5, 6, 3};
+    const R m = (R)p->mesh_margin;
+    const R m2 = m * m;
+    /* triangles whose bounds grown by a bit more than the margin miss x are
+       farther than the margin and cannot give a contact */
+    const R reach = m * (R)1.001 + (R)1e-6;
+    R best = m2;
+    int best_f = -1, best_feat = 0;
+    V3 best_q = F(v3)(0, 0, 0);
+    for (int f = 0; f < p->mesh_ntris; ++f) {
+        const float* t = p->mesh_tris + (size_t)f * HO_MESH_TRI_FLOATS;
+        int skip = 0;
+        for (int c = 0; c < 3 && !skip; ++c) {
+            R lo = FMIN(FMIN((R)t[c], (R)t[4 + c]), (R)t[8 + c]) - reach;
+            R hi = FMAX(FMAX((R)t[c], (R)t[4 + c]), (R)t[8 + c]) + reach;
+            R xc = c == 0 ? x.x : (c == 1 ? x.y : x.z);
+            skip = xc < lo || xc > hi;
+        }
+        if (skip) continue;
+        int feat;
+        V3 q = F(closest_on_tri)(x, F(v3)(t[0], t[1], t[2]), F(v3)(t[4], t[5], t[6]), F(v3)(t[8], t[9], t[10]),
+                                 &feat);
+        V3 dq = F(vsub)(x, q);
+        R d2 = F(dot)(dq, dq);
+        if (d2 < best) {
+            best = d2;
+            best_f = f;
+            best_feat = feat;
+            best_q = q;
+        }
+    }
+    if (best_f < 0) return 0;
+    const float* t = p->mesh_tris + (size_t)best_f * HO_MESH_TRI_FLOATS;
+    const float* n = t + 4 * slot_of_feature[best_feat];
+    V3 dq = F(vsub)(x, best_q);
+    if (!(F(dot)(dq, F(v3)(n[0], n[1], n[2])) < 0)) return 0;
+    R d = SQRT(best);
+    V3 dir = F(nrm)(F(vsub)(best_q, x));
+    *acc = F(vadd)(*acc, F(smul)(k * d, dir));
+    return 1;
+}
+
 /* Force at a start-of-step position: gravity (+ wind, + per-strand noise along
  * noise_dir) then the collider penalty loop, compute.comp:150-184. */
 static int F(force)(const ho_params* p, V3 x, R noise, V3* out)
@@ -164,6 +247,7 @@ static int F(force)(const ho_params* p, V3 x, R noise, V3* out)
             }
         }
     }
+    if (p->mesh_ntris > 0 && F(mesh_contact)(p, x, k, &acc)) hits += 1;
     if (hits > 0) {
         R m = (R)hits;
         f = F(vadd)(f, F(v3)(acc.x / m, acc.y / m, acc.z / m));

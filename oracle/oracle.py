@@ -36,7 +36,12 @@ class HoParams(C.Structure):
         ("seg_length", C.POINTER(C.c_float)),
         ("wind_noise", C.POINTER(C.c_float)),
         ("noise_dir", C.c_float * 3),
+        ("mesh_tris", C.POINTER(C.c_float)),
+        ("mesh_ntris", C.c_int),
+        ("mesh_margin", C.c_float),
     ]
+
+MESH_TRI_FLOATS = 40
 
 
 _lib = None
@@ -67,6 +72,8 @@ def lib():
         L.ho_force_f32.restype = C.c_int
         L.ho_force_f64.argtypes = [PP, DP, C.c_double, DP]
         L.ho_force_f64.restype = C.c_int
+        L.ho_mesh_prepare.argtypes = [FP, C.c_int, C.POINTER(C.c_uint32), C.c_int, FP]
+        L.ho_mesh_prepare.restype = C.c_int
         L.ho_sat_i32_f32.argtypes = [C.c_float]
         L.ho_sat_i32_f32.restype = C.c_int32
         L.ho_sat_i32_f64.argtypes = [C.c_double]
@@ -81,7 +88,7 @@ class Params:
     def __init__(self, *, dt, gravity=(0.0, -9.8, 0.0), wind=(0.0, 0.0, 0.0), damping=0.998, spring_k=1900.0,
                  sphere_radius=1.0, vmax=10.0, friction=0.08, hair_hair=False, grid_dim=0,
                  grid_origin=(0.0, 0.0, 0.0), grid_h=1.0, colliders=None, seg_length=None, wind_noise=None,
-                 noise_dir=(1.0, 0.0, 0.0)):
+                 noise_dir=(1.0, 0.0, 0.0), mesh=None, mesh_margin=0.0):
         p = HoParams()
         p.dt = dt
         p.gravity = (C.c_float * 3)(*gravity)
@@ -104,6 +111,11 @@ class Params:
         self._noise = None if wind_noise is None else np.ascontiguousarray(wind_noise, dtype=np.float32)
         p.wind_noise = None if self._noise is None else self._noise.ctypes.data_as(C.POINTER(C.c_float))
         p.noise_dir = (C.c_float * 3)(*noise_dir)
+        # mesh: the (T, 40) records of mesh_prepare
+        self._mesh = None if mesh is None else np.ascontiguousarray(mesh, dtype=np.float32)
+        p.mesh_tris = None if self._mesh is None else self._mesh.ctypes.data_as(C.POINTER(C.c_float))
+        p.mesh_ntris = 0 if self._mesh is None else self._mesh.shape[0]
+        p.mesh_margin = mesh_margin
         self.p = p
 
     def grid_buffer(self):
@@ -193,3 +205,16 @@ def force64(params, pos, noise=0.0):
 
 def sat_i32(f):
     return int(lib().ho_sat_i32_f32(f))
+
+
+def mesh_prepare(verts, tris):
+    """Per-triangle records (vertices + pseudo-normals) of the oracle's mesh collider."""
+    v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(tris, dtype=np.uint32).reshape(-1, 3)
+    out = np.zeros((t.shape[0], MESH_TRI_FLOATS), dtype=np.float32)
+    rc = lib().ho_mesh_prepare(v.ctypes.data_as(C.POINTER(C.c_float)), v.shape[0],
+                               t.ctypes.data_as(C.POINTER(C.c_uint32)), t.shape[0],
+                               out.ctypes.data_as(C.POINTER(C.c_float)))
+    if rc != 0:
+        raise ValueError("mesh is not a closed 2-manifold (or has an index out of range)")
+    return out

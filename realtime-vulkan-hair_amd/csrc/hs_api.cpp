@@ -21,6 +21,7 @@
 #include "hs_kernels.h"
 
 #include "hs_internal.h"
+#include "hs_mesh_build.h"
 
 using namespace hsi;
 
@@ -290,7 +291,12 @@ int build_args(hs_ctx* c, Groom& g, const hs_time* t, const hs_collider* cols, u
     if (p->num_colliders > ncol) return fail(c, HS_ERR_INVALID, w + ": params.num_colliders > colliders given");
     if (!(t->deltaTime > 0.0f) || !std::isfinite(t->deltaTime))
         return fail(c, HS_ERR_INVALID, w + ": deltaTime must be finite and > 0");
-    if (p->mesh_id != 0) return fail(c, HS_ERR_NOT_FOUND, w + ": unknown mesh");
+    const hsi::Mesh* mesh = nullptr;
+    if (p->mesh_id != 0) {
+        auto it = c->meshes.find(p->mesh_id);
+        if (it == c->meshes.end()) return fail(c, HS_ERR_NOT_FOUND, w + ": unknown mesh");
+        mesh = &it->second;
+    }
     hs::StepArgs& a = *out;
     std::memset(&a, 0, sizeof(a));
     a.pos = g.pos;
@@ -327,6 +333,7 @@ int build_args(hs_ctx* c, Groom& g, const hs_time* t, const hs_collider* cols, u
     a.ndz = g.noise_dir[2];
     a.ncol = p->num_colliders;
     a.cols = g.cols;
+    if (mesh) a.mesh = mesh->args;
     if (a.ncol > 0) {
         a.sphere_c[0] = cols[0].transform[12];
         a.sphere_c[1] = cols[0].transform[13];
@@ -403,8 +410,10 @@ int ensure_blob(hs_ctx* c, size_t bytes)
 struct BatchLaunch {
     uint64_t args_off, wg_off, br_off;
     uint32_t ng, total_wg, total_br;
-    uint8_t v;  // bit 0 gather, bit 1 splat, bit 2 ellipsoids
+    uint8_t v;  // bit 0 gather, bit 1 splat, bit 2 ellipsoids, bit 3 mesh
 };
+
+int collider_mode(uint8_t v) { return ((v & 4) ? hs::kColEll : 0) | ((v & 8) ? hs::kColMesh : 0); }
 
 hipError_t enqueue_batch(const hs_ctx* c, const std::vector<BatchLaunch>& launches, size_t bytes, int slot,
                          hipStream_t st)
@@ -414,7 +423,7 @@ hipError_t enqueue_batch(const hs_ctx* c, const std::vector<BatchLaunch>& launch
     for (const BatchLaunch& L : launches) {
         e = hs::launch_step_batch(reinterpret_cast<const hs::StepArgs*>(c->dev_blob + L.args_off),
                                   reinterpret_cast<const uint32_t*>(c->dev_blob + L.wg_off), L.ng, L.total_wg,
-                                  (L.v & 1) != 0, (L.v & 2) != 0, (L.v & 4) != 0, st);
+                                  (L.v & 1) != 0, (L.v & 2) != 0, collider_mode(L.v), st);
         if (e != hipSuccess) return e;
     }
     for (const BatchLaunch& L : launches) {
@@ -534,6 +543,8 @@ void hs_destroy(hs_ctx* c)
     (void)hipStreamSynchronize(c->stream);
     for (auto& kv : c->grooms) free_groom(c, kv.second);
     c->grooms.clear();
+    for (auto& kv : c->meshes) (void)hipFree(kv.second.block);
+    c->meshes.clear();
     for (auto& ev : c->events)
         for (int k = 0; k < 3; ++k) (void)hipEventDestroy(ev.e[k]);
     drop_graphs(c);
@@ -715,6 +726,74 @@ int hs_groom_create(hs_ctx* c, const hs_groom_desc* d, const float* aos, uint32_
     return HS_OK;
 }
 
+int hs_mesh_create(hs_ctx* c, const float* v, uint32_t nv, const uint32_t* t, uint32_t nt, const hs_mesh_desc* d,
+                   uint32_t* id)
+{
+    if (!c) return HS_ERR_INVALID;
+    if (!v || !t || !d || !id) return fail(c, HS_ERR_INVALID, "hs_mesh_create: null argument");
+    if (!(d->margin > 0.0f) || !std::isfinite(d->margin))
+        return fail(c, HS_ERR_INVALID, "hs_mesh_create: margin must be finite and > 0");
+    for (int k = 0; k < 3; ++k)
+        if (d->grid_dim[k] == 0 || d->grid_dim[k] > 1024)
+            return fail(c, HS_ERR_INVALID, "hs_mesh_create: grid_dim must be in [1, 1024]");
+    hsmesh::Built b;
+    if (!hsmesh::build(v, nv, t, nt, d->grid_dim, d->margin, &b))
+        return fail(c, HS_ERR_INVALID, "hs_mesh_create: mesh is not closed (or an index is out of range)");
+    HS_TRY(c, hipSetDevice(c->device));
+    const uint64_t tri_bytes = round_up(b.tris.size() * 4, 256);
+    const uint64_t start_bytes = round_up(b.cell_start.size() * 4, 256);
+    const uint64_t ids_bytes = round_up(std::max<size_t>(b.cell_tris.size(), 1) * 4, 256);
+    hsi::Mesh m;
+    m.bytes = tri_bytes + start_bytes + ids_bytes;
+    m.ntris = nt;
+    HS_TRY(c, hipMalloc(&m.block, m.bytes));
+    uint8_t* base = static_cast<uint8_t*>(m.block);
+    hipError_t e = hipMemcpyAsync(base, b.tris.data(), b.tris.size() * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(base + tri_bytes, b.cell_start.data(), b.cell_start.size() * 4, hipMemcpyHostToDevice,
+                           c->stream);
+    if (e == hipSuccess && !b.cell_tris.empty())
+        e = hipMemcpyAsync(base + tri_bytes + start_bytes, b.cell_tris.data(), b.cell_tris.size() * 4,
+                           hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(m.block);
+        return hip_fail(c, e, "hs_mesh_create: upload");
+    }
+    hs::MeshArgs& a = m.args;
+    a.tris = reinterpret_cast<const float4*>(base);
+    a.cell_start = reinterpret_cast<const uint32_t*>(base + tri_bytes);
+    a.cell_tris = reinterpret_cast<const uint32_t*>(base + tri_bytes + start_bytes);
+    a.ox = b.origin[0];
+    a.oy = b.origin[1];
+    a.oz = b.origin[2];
+    a.icx = b.inv_cell[0];
+    a.icy = b.inv_cell[1];
+    a.icz = b.inv_cell[2];
+    a.dx = b.dim[0];
+    a.dy = b.dim[1];
+    a.dz = b.dim[2];
+    a.margin2 = d->margin * d->margin;
+    const uint32_t mid = c->next_mesh++;
+    c->meshes[mid] = m;
+    c->bytes += m.bytes;
+    *id = mid;
+    return HS_OK;
+}
+
+int hs_mesh_destroy(hs_ctx* c, uint32_t id)
+{
+    if (!c) return HS_ERR_INVALID;
+    auto it = c->meshes.find(id);
+    if (it == c->meshes.end()) return fail(c, HS_ERR_NOT_FOUND, "hs_mesh_destroy: unknown mesh");
+    HS_TRY(c, hipSetDevice(c->device));
+    HS_TRY(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(it->second.block);
+    c->bytes -= it->second.bytes;
+    c->meshes.erase(it);
+    return HS_OK;
+}
+
 int hs_groom_destroy(hs_ctx* c, uint32_t id)
 {
     if (!c) return HS_ERR_INVALID;
@@ -786,20 +865,20 @@ int hs_step_batch(hs_ctx* c, const hs_batch_item* items, uint32_t n)
                             "hs_step_batch", &args[k], &gather, &splat, stage);
         if (rc != HS_OK) return rc;
         const bool e = args[k].ncol > 1;
-        variant[k] = (uint8_t)((gather ? 1 : 0) | (splat ? 2 : 0) | (e ? 4 : 0));
+        variant[k] = (uint8_t)((gather ? 1 : 0) | (splat ? 2 : 0) | (e ? 4 : 0) | (args[k].mesh.tris ? 8 : 0));
         if (e) {
             ell_at[k] = (uint32_t)ell.size();
             ell.insert(ell.end(), stage, stage + args[k].ncol);
         }
     }
     // blob: [StepArgs of every group][ellipsoids][prefix arrays of every group]
-    std::vector<std::vector<uint32_t>> groups(8);
+    std::vector<std::vector<uint32_t>> groups(16);
     for (uint32_t k = 0; k < n; ++k) groups[variant[k]].push_back(k);
     std::vector<uint32_t> order;
     std::vector<uint32_t> idx;
     std::vector<BatchLaunch> launches;
     std::vector<uint64_t> key;
-    for (uint8_t v = 0; v < 8; ++v) {
+    for (uint8_t v = 0; v < 16; ++v) {
         if (groups[v].empty()) continue;
         BatchLaunch L{order.size() * sizeof(hs::StepArgs), 0, 0, (uint32_t)groups[v].size(), 0, 0, v};
         std::vector<uint32_t> wg{0}, br{0};
@@ -866,7 +945,7 @@ int hs_step_batch(hs_ctx* c, const hs_batch_item* items, uint32_t n)
         for (const BatchLaunch& L : launches)
             HS_TRY(c, hs::launch_step_batch(reinterpret_cast<const hs::StepArgs*>(c->dev_blob + L.args_off),
                                             reinterpret_cast<const uint32_t*>(c->dev_blob + L.wg_off), L.ng,
-                                            L.total_wg, (L.v & 1) != 0, (L.v & 2) != 0, (L.v & 4) != 0, c->stream));
+                                            L.total_wg, (L.v & 1) != 0, (L.v & 2) != 0, collider_mode(L.v), c->stream));
         HS_TRY(c, hipEventRecord(ev->e[1], c->stream));
         for (const BatchLaunch& L : launches)
             if (L.v & 2)

@@ -78,6 +78,13 @@ void groom_release(Groom& g);
 // memory `aos_dev` (48*S*N bytes), on stream `st`.
 hipError_t groom_unpack(const Groom& g, float* aos_dev, hipStream_t st);
 
+// A mesh collider on the device (hs_mesh_create).
+struct Mesh {
+    void* block = nullptr;
+    uint64_t bytes = 0;
+    uint32_t ntris = 0;
+    hs::MeshArgs args{};
+};
 }  // namespace hsi
 
 struct hs_ctx {
@@ -88,6 +95,8 @@ struct hs_ctx {
     std::map<uint32_t, hsi::Groom> grooms;
     uint32_t next_id = 1;
     uint64_t bytes = 0;
+    std::map<uint32_t, hsi::Mesh> meshes;
+    uint32_t next_mesh = 1;
     // kernel timing: pooled events (step start, step end, reduce end) per step
     bool profile = false;
     struct Ev {

@@ -47,6 +47,23 @@ struct GridWork {
 };
 
 // Everything one step launch needs, passed by value as the kernel argument.
+// Triangle-mesh collider: per-triangle records of 10 float4 (a, b, c, face
+// normal, edge pseudo-normals ab bc ca, vertex pseudo-normals a b c) and a
+// uniform grid of ascending triangle-id lists.
+struct MeshArgs {
+    const float4* tris;
+    const uint32_t* cell_start;  // [cells + 1]
+    const uint32_t* cell_tris;
+    float ox, oy, oz;            // grid origin
+    float icx, icy, icz;         // 1 / cell size
+    uint32_t dx, dy, dz;         // cells per axis
+    float margin2;               // contact when |x - q|^2 < margin2
+};
+
+// Collider mode of a step kernel: which colliders beyond the sphere it handles.
+constexpr int kColEll = 1;
+constexpr int kColMesh = 2;
+
 struct StepArgs {
     float* pos;           // [3][N][Sp]
     float* vel;           // [3][N][Sp]
@@ -68,6 +85,7 @@ struct StepArgs {
     float sphere_c[3];    // collider 0 centre (Collider.transform[3].xyz)
     uint32_t ncol;        // colliders in use (0: none, 1: the sphere, >1: + ellipsoids)
     const Collider* cols; // device copy of all ncol colliders (read for ellipsoids 1..ncol-1)
+    MeshArgs mesh;        // triangle-mesh collider (mesh.tris null: none)
 };
 
 struct UnpackGrid {
@@ -81,7 +99,7 @@ hipError_t launch_reduce(const GridWork& gw, uint32_t G, uint32_t parity, hipStr
 // Many grooms per launch: per-groom arguments in device memory; wg_start /
 // brick_start are prefix sums (ngroom + 1 entries) of workgroups / bricks.
 hipError_t launch_step_batch(const StepArgs* args_dev, const uint32_t* wg_start_dev, uint32_t ngroom,
-                             uint32_t total_wg, bool gather, bool splat, bool ell, hipStream_t st);
+                             uint32_t total_wg, bool gather, bool splat, int col, hipStream_t st);
 hipError_t launch_reduce_batch(const StepArgs* args_dev, const uint32_t* brick_start_dev, uint32_t ngroom,
                                uint32_t total_bricks, hipStream_t st);
 hipError_t launch_pack(const float* aos_dev, const uint32_t* perm, uint32_t S, uint32_t N, uint32_t Sp,

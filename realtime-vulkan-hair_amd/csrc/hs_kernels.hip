@@ -306,7 +306,104 @@ __device__ __forceinline__ void splat_any(int4* win, const Win& wd, bool lds, in
 // Force at a start-of-step position: gravity (+ wind, + strand noise) then the
 // collider penalty loop (compute.comp:150-184).
 // ---------------------------------------------------------------------------
-template <bool ELL>
+// Closest point on triangle abc to x (Ericson, Real-Time Collision Detection,
+// 2005, 5.1.5) and the feature it lies on: 0..2 vertex a/b/c, 3 edge ab,
+// 4 edge bc, 5 edge ca, 6 face.
+__device__ __forceinline__ f3 closest_on_tri(f3 x, f3 pa, f3 pb, f3 pc, int* feature)
+{
+    const f3 ab = sub(pb, pa), ac = sub(pc, pa), ap = sub(x, pa);
+    const float d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.0f && d2 <= 0.0f) {
+        *feature = 0;
+        return pa;
+    }
+    const f3 bp = sub(x, pb);
+    const float d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0.0f && d4 <= d3) {
+        *feature = 1;
+        return pb;
+    }
+    const float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+        *feature = 3;
+        return add(pa, smul(d1 / (d1 - d3), ab));
+    }
+    const f3 cp = sub(x, pc);
+    const float d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0.0f && d5 <= d6) {
+        *feature = 2;
+        return pc;
+    }
+    const float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+        *feature = 5;
+        return add(pa, smul(d2 / (d2 - d6), ac));
+    }
+    const float va = d3 * d6 - d5 * d4;
+    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+        *feature = 4;
+        return add(pb, smul((d4 - d3) / ((d4 - d3) + (d5 - d6)), sub(pc, pb)));
+    }
+    const float denom = 1.0f / ((va + vb) + vc);
+    const float v = vb * denom, w = vc * denom;
+    *feature = 6;
+    return add(add(pa, smul(v, ab)), smul(w, ac));
+}
+
+__device__ __forceinline__ f3 ld_f3(const float4* p)
+{
+    const float4 v = *p;
+    return mk(v.x, v.y, v.z);
+}
+
+// Mesh contact: the mesh point q nearest to x among the triangles listed for
+// x's grid cell (every triangle within the margin of the cell is listed, so
+// within the margin this is the nearest over the whole mesh; lowest id wins
+// ties); contact when |x - q|^2 < margin^2 and dot(x - q, n) < 0 for the
+// pseudo-normal n of q's feature (Baerentzen & Aanaes, IEEE TVCG 2005).
+// Returns (k |x - q|) normalize(q - x) in xyz and 1 in w on contact, else 0.
+// No reference counterpart.  Not inlined: one copy serves every point.
+__device__ __noinline__ float4 mesh_contact(MeshArgs M, float px, float py, float pz, float k)
+{
+    const f3 x = mk(px, py, pz);
+    const float4 none = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float gx = (x.x - M.ox) * M.icx, gy = (x.y - M.oy) * M.icy, gz = (x.z - M.oz) * M.icz;
+    // outside the grid (or NaN): farther than the margin from every triangle
+    if (!(gx >= 0.0f && gx < (float)M.dx && gy >= 0.0f && gy < (float)M.dy && gz >= 0.0f && gz < (float)M.dz))
+        return none;
+    const uint32_t cx = min((uint32_t)gx, M.dx - 1), cy = min((uint32_t)gy, M.dy - 1),
+                   cz = min((uint32_t)gz, M.dz - 1);
+    const uint32_t cell = (cz * M.dy + cy) * M.dx + cx;
+    const uint32_t i0 = M.cell_start[cell], i1 = M.cell_start[cell + 1];
+    float best = M.margin2;
+    int best_feat = 0;
+    uint32_t best_t = 0xffffffffu;
+    f3 best_q = mk(0.0f, 0.0f, 0.0f);
+    for (uint32_t i = i0; i < i1; ++i) {
+        const uint32_t t = M.cell_tris[i];
+        const float4* r = M.tris + (size_t)t * 10;
+        int feat;
+        const f3 q = closest_on_tri(x, ld_f3(r), ld_f3(r + 1), ld_f3(r + 2), &feat);
+        const f3 dq = sub(x, q);
+        const float d2 = dot(dq, dq);
+        if (d2 < best) {
+            best = d2;
+            best_t = t;
+            best_feat = feat;
+            best_q = q;
+        }
+    }
+    if (best_t == 0xffffffffu) return none;
+    // feature -> record slot: vertices 7..9, edges 4..6, face 3
+    const int slot = best_feat < 3 ? 7 + best_feat : (best_feat < 6 ? best_feat + 1 : 3);
+    const f3 n = ld_f3(M.tris + (size_t)best_t * 10 + slot);
+    if (!(dot(sub(x, best_q), n) < 0.0f)) return none;
+    const float d = sqrtf(best);
+    const f3 c = smul(k * d, normalize(sub(best_q, x)));
+    return make_float4(c.x, c.y, c.z, 1.0f);
+}
+
+template <int COL>
 __device__ __forceinline__ f3 force_at(f3 x, f3 fs, const StepArgs& a)
 {
     f3 acc = mk(0.0f, 0.0f, 0.0f);
@@ -324,7 +421,7 @@ __device__ __forceinline__ f3 force_at(f3 x, f3 fs, const StepArgs& a)
             hits = 1;
         }
     }
-    if (ELL) {
+    if (COL & kColEll) {
         // ellipsoids: EllipsoidCollision / GetPointOnEllipsoid /
         // GetEllipsoidNormal (compute.comp:64-79, 170-179)
 #pragma unroll 1
@@ -338,6 +435,14 @@ __device__ __forceinline__ f3 force_at(f3 x, f3 fs, const StepArgs& a)
                 acc = add(acc, smul(a.k * d, n));
                 hits += 1;
             }
+        }
+    }
+    if (COL & kColMesh) {
+        // the mesh counts as one more collider, after the reference's ones
+        const float4 c = mesh_contact(a.mesh, x.x, x.y, x.z, a.k);
+        if (c.w != 0.0f) {
+            acc = add(acc, mk(c.x, c.y, c.z));
+            hits += 1;
         }
     }
     f3 force = fs;
@@ -492,7 +597,7 @@ __device__ __forceinline__ uint32_t xcd_swizzle(uint32_t block, uint32_t nblocks
 }
 
 // One workgroup's share of a step: strands [wg*kW, wg*kW + kW) of the groom `a`.
-template <bool GATHER, bool SPLAT, bool ELL>
+template <bool GATHER, bool SPLAT, int COL>
 __device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
 {
     __shared__ __align__(16) int4 win[kWinCells];
@@ -602,7 +707,7 @@ __device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
                     u = friction(u, gv, a.mu_prev, a.omu_prev);
                 }
                 // predict (compute.comp:187)
-                const f3 fc = force_at<ELL>(x, fs, a);
+                const f3 fc = force_at<COL>(x, fs, a);
                 U[t] = add(add(x, smul(a.dt, u)), smul(a.dt2, fc));
             }
         }
@@ -834,12 +939,12 @@ __device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
     }
 }
 
-template <bool GATHER, bool SPLAT, bool ELL>
+template <bool GATHER, bool SPLAT, int COL>
 __global__ __launch_bounds__(kW) void k_step(StepArgs a)
 {
     const uint32_t wg = xcd_swizzle(blockIdx.x, a.nwg);
     if (wg >= a.nwg) return;
-    step_body<GATHER, SPLAT, ELL>(a, wg);
+    step_body<GATHER, SPLAT, COL>(a, wg);
 }
 
 // Many grooms in one launch: wg_start[g] is the first global workgroup of
@@ -857,7 +962,7 @@ __device__ __forceinline__ uint32_t find_groom(const uint32_t* __restrict__ star
     return lo;
 }
 
-template <bool GATHER, bool SPLAT, bool ELL>
+template <bool GATHER, bool SPLAT, int COL>
 __global__ __launch_bounds__(kW) void k_step_batch(const StepArgs* __restrict__ args,
                                                    const uint32_t* __restrict__ wg_start, uint32_t ngroom)
 {
@@ -867,7 +972,7 @@ __global__ __launch_bounds__(kW) void k_step_batch(const StepArgs* __restrict__ 
     const uint32_t g = find_groom(wg_start, ngroom, gwg);
     // by-value copy: the arguments then live in SGPRs like a kernel argument
     const StepArgs a = args[g];
-    step_body<GATHER, SPLAT, ELL>(a, gwg - wg_start[g]);
+    step_body<GATHER, SPLAT, COL>(a, gwg - wg_start[g]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1105,56 +1210,71 @@ __global__ void k_unpack(float4* __restrict__ aos, const uint32_t* __restrict__ 
 // Host-side launchers
 // ---------------------------------------------------------------------------
 template <bool G, bool S>
-static void launch_step_e(const StepArgs& a, bool ell, uint32_t blocks, hipStream_t st)
+static void launch_step_e(const StepArgs& a, int col, uint32_t blocks, hipStream_t st)
 {
-    if (ell)
-        hipLaunchKernelGGL((k_step<G, S, true>), dim3(blocks), dim3(kW), 0, st, a);
-    else
-        hipLaunchKernelGGL((k_step<G, S, false>), dim3(blocks), dim3(kW), 0, st, a);
+    switch (col) {
+    case 0: hipLaunchKernelGGL((k_step<G, S, 0>), dim3(blocks), dim3(kW), 0, st, a); break;
+    case kColEll: hipLaunchKernelGGL((k_step<G, S, kColEll>), dim3(blocks), dim3(kW), 0, st, a); break;
+    case kColMesh: hipLaunchKernelGGL((k_step<G, S, kColMesh>), dim3(blocks), dim3(kW), 0, st, a); break;
+    default:
+        hipLaunchKernelGGL((k_step<G, S, kColEll | kColMesh>), dim3(blocks), dim3(kW), 0, st, a);
+        break;
+    }
 }
 
 hipError_t launch_step(const StepArgs& a, bool gather, bool splat, hipStream_t st)
 {
     const uint32_t blocks = ((a.nwg + 7) / 8) * 8;
-    const bool ell = a.ncol > 1;
+    const int col = (a.ncol > 1 ? kColEll : 0) | (a.mesh.tris ? kColMesh : 0);
     if (gather) {
         if (splat)
-            launch_step_e<true, true>(a, ell, blocks, st);
+            launch_step_e<true, true>(a, col, blocks, st);
         else
-            launch_step_e<true, false>(a, ell, blocks, st);
+            launch_step_e<true, false>(a, col, blocks, st);
     } else {
         if (splat)
-            launch_step_e<false, true>(a, ell, blocks, st);
+            launch_step_e<false, true>(a, col, blocks, st);
         else
-            launch_step_e<false, false>(a, ell, blocks, st);
+            launch_step_e<false, false>(a, col, blocks, st);
     }
     return hipGetLastError();
 }
 
 template <bool G, bool S>
-static void launch_batch_e(const StepArgs* args, const uint32_t* wg_start, uint32_t ngroom, bool ell,
+static void launch_batch_e(const StepArgs* args, const uint32_t* wg_start, uint32_t ngroom, int col,
                            uint32_t blocks, hipStream_t st)
 {
-    if (ell)
-        hipLaunchKernelGGL((k_step_batch<G, S, true>), dim3(blocks), dim3(kW), 0, st, args, wg_start, ngroom);
-    else
-        hipLaunchKernelGGL((k_step_batch<G, S, false>), dim3(blocks), dim3(kW), 0, st, args, wg_start, ngroom);
+    switch (col) {
+    case 0:
+        hipLaunchKernelGGL((k_step_batch<G, S, 0>), dim3(blocks), dim3(kW), 0, st, args, wg_start, ngroom);
+        break;
+    case kColEll:
+        hipLaunchKernelGGL((k_step_batch<G, S, kColEll>), dim3(blocks), dim3(kW), 0, st, args, wg_start, ngroom);
+        break;
+    case kColMesh:
+        hipLaunchKernelGGL((k_step_batch<G, S, kColMesh>), dim3(blocks), dim3(kW), 0, st, args, wg_start, ngroom);
+        break;
+    default:
+        hipLaunchKernelGGL((k_step_batch<G, S, kColEll | kColMesh>), dim3(blocks), dim3(kW), 0, st, args, wg_start,
+                           ngroom);
+        break;
+    }
 }
 
 hipError_t launch_step_batch(const StepArgs* args_dev, const uint32_t* wg_start_dev, uint32_t ngroom,
-                             uint32_t total_wg, bool gather, bool splat, bool ell, hipStream_t st)
+                             uint32_t total_wg, bool gather, bool splat, int col, hipStream_t st)
 {
     const uint32_t blocks = ((total_wg + 7) / 8) * 8;
     if (gather) {
         if (splat)
-            launch_batch_e<true, true>(args_dev, wg_start_dev, ngroom, ell, blocks, st);
+            launch_batch_e<true, true>(args_dev, wg_start_dev, ngroom, col, blocks, st);
         else
-            launch_batch_e<true, false>(args_dev, wg_start_dev, ngroom, ell, blocks, st);
+            launch_batch_e<true, false>(args_dev, wg_start_dev, ngroom, col, blocks, st);
     } else {
         if (splat)
-            launch_batch_e<false, true>(args_dev, wg_start_dev, ngroom, ell, blocks, st);
+            launch_batch_e<false, true>(args_dev, wg_start_dev, ngroom, col, blocks, st);
         else
-            launch_batch_e<false, false>(args_dev, wg_start_dev, ngroom, ell, blocks, st);
+            launch_batch_e<false, false>(args_dev, wg_start_dev, ngroom, col, blocks, st);
     }
     return hipGetLastError();
 }

@@ -1,0 +1,141 @@
+// hs_mesh_build.h - host preprocessing of a closed triangle mesh collider:
+// per-triangle records (vertices, face/edge/vertex pseudo-normals) and a
+// uniform grid of triangle lists, uploaded by hs_mesh_create.
+//
+// The reference has no mesh collider (compute.comp:155-184 handles a sphere
+// and ellipsoids only); the contact rule this serves is stated with the
+// kernel (hs_kernels.hip, mesh_contact).
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <map>
+#include <utility>
+#include <vector>
+
+namespace hsmesh {
+
+// One triangle: vertices a, b, c; the face normal; the pseudo-normals of
+// edges ab, bc, ca; the pseudo-normals of vertices a, b, c (angle-weighted,
+// Baerentzen & Aanaes 2005).  10 float4 (w unused) = 160 B.
+constexpr int kTriFloats = 40;
+
+struct Built {
+    std::vector<float> tris;          // [T][kTriFloats]
+    std::vector<uint32_t> cell_start; // [cells + 1]
+    std::vector<uint32_t> cell_tris;  // triangle ids per cell, ascending
+    float origin[3] = {0, 0, 0};
+    float cell[3] = {1, 1, 1};
+    float inv_cell[3] = {1, 1, 1};
+    uint32_t dim[3] = {1, 1, 1};
+    float margin = 0.0f;
+};
+
+struct D3 {
+    double x, y, z;
+};
+inline D3 sub(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline D3 add(D3 a, D3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline D3 mul(D3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+inline double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline D3 cross(D3 a, D3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline D3 unit(D3 a)
+{
+    const double l = std::sqrt(dot(a, a));
+    return l > 0 ? mul(a, 1.0 / l) : D3{0, 0, 0};
+}
+
+// Returns false for a mesh that is not a closed 2-manifold (an edge not
+// shared by exactly two triangles) or has an index out of range.
+inline bool build(const float* v, uint32_t nv, const uint32_t* t, uint32_t nt, const uint32_t dims[3], float margin,
+                  Built* out)
+{
+    if (nv < 4 || nt < 4) return false;
+    for (uint64_t k = 0; k < 3ull * nt; ++k)
+        if (t[k] >= nv) return false;
+    auto V = [&](uint32_t i) { return D3{v[3 * i], v[3 * i + 1], v[3 * i + 2]}; };
+    std::vector<D3> fn(nt), vn(nv, D3{0, 0, 0});
+    std::map<std::pair<uint32_t, uint32_t>, std::vector<uint32_t>> edges;
+    for (uint32_t f = 0; f < nt; ++f) {
+        const uint32_t i[3] = {t[3 * f], t[3 * f + 1], t[3 * f + 2]};
+        const D3 p[3] = {V(i[0]), V(i[1]), V(i[2])};
+        fn[f] = unit(cross(sub(p[1], p[0]), sub(p[2], p[0])));
+        for (int k = 0; k < 3; ++k) {
+            const D3 e1 = unit(sub(p[(k + 1) % 3], p[k]));
+            const D3 e2 = unit(sub(p[(k + 2) % 3], p[k]));
+            const double ang = std::acos(std::max(-1.0, std::min(1.0, dot(e1, e2))));
+            vn[i[k]] = add(vn[i[k]], mul(fn[f], ang));
+            const uint32_t a = i[k], b = i[(k + 1) % 3];
+            edges[{std::min(a, b), std::max(a, b)}].push_back(f);
+        }
+    }
+    for (auto& e : edges)
+        if (e.second.size() != 2) return false;
+    Built& B = *out;
+    B.tris.assign((size_t)nt * kTriFloats, 0.0f);
+    D3 lo{1e300, 1e300, 1e300}, hi{-1e300, -1e300, -1e300};
+    for (uint32_t f = 0; f < nt; ++f) {
+        const uint32_t i[3] = {t[3 * f], t[3 * f + 1], t[3 * f + 2]};
+        float* r = &B.tris[(size_t)f * kTriFloats];
+        auto put = [&](int slot, D3 x) {
+            r[4 * slot] = (float)x.x;
+            r[4 * slot + 1] = (float)x.y;
+            r[4 * slot + 2] = (float)x.z;
+        };
+        for (int k = 0; k < 3; ++k) {
+            const D3 p = V(i[k]);
+            put(k, p);
+            lo = {std::min(lo.x, p.x), std::min(lo.y, p.y), std::min(lo.z, p.z)};
+            hi = {std::max(hi.x, p.x), std::max(hi.y, p.y), std::max(hi.z, p.z)};
+        }
+        put(3, fn[f]);
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t a = i[k], b = i[(k + 1) % 3];
+            const auto& adj = edges[{std::min(a, b), std::max(a, b)}];
+            put(4 + k, unit(add(fn[adj[0]], fn[adj[1]])));
+            put(7 + k, unit(vn[i[k]]));
+        }
+    }
+    // grid over the mesh bounds grown by the margin
+    const double m = margin;
+    lo = sub(lo, D3{m, m, m});
+    hi = add(hi, D3{m, m, m});
+    const double ext[3] = {hi.x - lo.x, hi.y - lo.y, hi.z - lo.z};
+    for (int k = 0; k < 3; ++k) {
+        B.dim[k] = std::max<uint32_t>(1, dims[k]);
+        B.cell[k] = (float)(ext[k] / B.dim[k]);
+        B.inv_cell[k] = 1.0f / B.cell[k];
+    }
+    B.origin[0] = (float)lo.x;
+    B.origin[1] = (float)lo.y;
+    B.origin[2] = (float)lo.z;
+    B.margin = margin;
+    const uint64_t ncell = (uint64_t)B.dim[0] * B.dim[1] * B.dim[2];
+    std::vector<std::vector<uint32_t>> lists(ncell);
+    // a cell lists every triangle whose bounds grown by a bit more than the
+    // margin reach it (the extra absorbs float rounding of the point's cell)
+    const double reach = m * 1.001 + 1e-6 + 1e-5 * std::max({ext[0], ext[1], ext[2]}) / 1024.0;
+    for (uint32_t f = 0; f < nt; ++f) {
+        const float* r = &B.tris[(size_t)f * kTriFloats];
+        int c0[3], c1[3];
+        for (int k = 0; k < 3; ++k) {
+            const double tmin = std::min({(double)r[k], (double)r[4 + k], (double)r[8 + k]}) - reach;
+            const double tmax = std::max({(double)r[k], (double)r[4 + k], (double)r[8 + k]}) + reach;
+            c0[k] = std::max(0, (int)std::floor((tmin - B.origin[k]) / B.cell[k]));
+            c1[k] = std::min((int)B.dim[k] - 1, (int)std::floor((tmax - B.origin[k]) / B.cell[k]));
+        }
+        for (int z = c0[2]; z <= c1[2]; ++z)
+            for (int y = c0[1]; y <= c1[1]; ++y)
+                for (int x = c0[0]; x <= c1[0]; ++x)
+                    lists[((uint64_t)z * B.dim[1] + y) * B.dim[0] + x].push_back(f);
+    }
+    B.cell_start.assign(ncell + 1, 0);
+    for (uint64_t c = 0; c < ncell; ++c) B.cell_start[c + 1] = B.cell_start[c] + (uint32_t)lists[c].size();
+    B.cell_tris.clear();
+    B.cell_tris.reserve(B.cell_start[ncell]);
+    for (auto& l : lists) B.cell_tris.insert(B.cell_tris.end(), l.begin(), l.end());
+    return true;
+}
+
+}  // namespace hsmesh

@@ -99,6 +99,10 @@ class HsStepParams(C.Structure):
     ]
 
 
+class HsMeshDesc(C.Structure):
+    _fields_ = [("grid_dim", C.c_uint32 * 3), ("margin", C.c_float)]
+
+
 class HsBatchItem(C.Structure):
     _fields_ = [
         ("groom_id", C.c_uint32),
@@ -164,6 +168,9 @@ _SIGS = {
     "hs_grid_near_limit": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint64)]),
     "hs_set_batch_graphs": (C.c_int, [_P, C.c_int]),
     "hs_batch_stats": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "hs_mesh_create": (C.c_int, [_P, C.POINTER(C.c_float), C.c_uint32, C.POINTER(C.c_uint32), C.c_uint32,
+                                 C.POINTER(HsMeshDesc), C.POINTER(C.c_uint32)]),
+    "hs_mesh_destroy": (C.c_int, [_P, C.c_uint32]),
     "hs_profile_enable": (C.c_int, [_P, C.c_int]),
     "hs_profile_read": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
     "hs_profile_breakdown": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
@@ -328,6 +335,19 @@ class Context:
 
     def batch(self, grooms, max_colliders=1):
         return Batch(self, grooms, max_colliders)
+
+    def mesh(self, verts, tris, grid_dim=(16, 16, 16), margin=0.05):
+        """Register a closed triangle mesh collider; returns its id for step_params(mesh_id=...)."""
+        v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
+        t = np.ascontiguousarray(tris, dtype=np.uint32).reshape(-1, 3)
+        d = HsMeshDesc((C.c_uint32 * 3)(*grid_dim), margin)
+        mid = C.c_uint32()
+        self._check(lib().hs_mesh_create(self._h, _fptr(v), v.shape[0], t.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                         t.shape[0], C.byref(d), C.byref(mid)))
+        return mid.value
+
+    def mesh_destroy(self, mesh_id):
+        self._check(lib().hs_mesh_destroy(self._h, mesh_id))
 
     def profile_breakdown(self):
         a = C.c_double()

@@ -97,6 +97,9 @@ class Workload:
     sphere_orbit_y: float = 0.5
     sphere_period: float = 4.0
     sphere_centers: list = field(default_factory=list)  # per-groom fixed centres
+    mesh: tuple = None             # (verts (V,3) f32, tris (T,3) u32) head mesh collider, or None
+    mesh_margin: float = 0.05
+    mesh_grid: tuple = (32, 32, 32)
 
     def time(self, k):
         """totalTime seen by step k (0-based)."""
@@ -142,11 +145,31 @@ def plain_100k(seed=SEED_BASE + 1, S=100_000, N=32, G=64):
                     sphere_radius=0.3, grooms=[g], wind_amp=2.0, wind_period=1.0, sphere_orbit=1.1)
 
 
+def head_mesh(subdiv=5, scale=(1.0, 1.2, 1.0)):
+    """The synthetic head: an icosphere (subdivision 5: 10,242 vertices,
+    20,480 triangles) scaled per axis."""
+    return icosphere(subdiv, scale)
+
+
+def mesh_head_scene(name, seed, S, N, G, subdiv, scale=(1.0, 1.2, 1.0), margin=0.05, mesh_grid=(32, 38, 32),
+                    steps=120):
+    """Roots on the scaled upper hemisphere (the head's scalp), strands along
+    the surface normal; wind, noise, orbiting sphere and grid as plain-100k,
+    plus the head mesh collider."""
+    rng = np.random.default_rng(seed)
+    roots, nrm = hemisphere_roots(rng, S, scale=scale)
+    L = rng.uniform(0.8, 1.2, S)
+    aos, seg = straight_strands(roots, nrm, N, L)
+    noise = (rng.standard_normal(S) * 0.1).astype(np.float32)
+    g = Groom(aos, N, seg, noise, grid_dim=G, grid_origin=(-2.2, -2.2, -2.2), grid_h=4.4 / G)
+    return Workload(name, dt=1.0 / 60.0, steps=steps, damping=0.9, friction=0.05, hair_hair=True,
+                    sphere_radius=0.3, grooms=[g], wind_amp=2.0, wind_period=1.0, sphere_orbit=1.1,
+                    mesh=icosphere(subdiv, scale), mesh_margin=margin, mesh_grid=mesh_grid)
+
+
 def large_1m(seed=SEED_BASE + 4, S=1_000_000, N=64, G=128):
-    w = plain_100k(seed=seed, S=S, N=N, G=G)
-    w.name = "large-1m"
-    w.steps = 120
-    return w
+    """1,000,000 strands x 64 points, 128^3 grid, head mesh of 20,480 triangles."""
+    return mesh_head_scene("large-1m", seed, S, N, G, subdiv=5)
 
 
 def batched(seed=SEED_BASE + 2, n_grooms=256, s_lo=512, s_hi=16384, n_choices=(8, 16, 32, 64), G=32):
@@ -164,6 +187,36 @@ def batched(seed=SEED_BASE + 2, n_grooms=256, s_lo=512, s_hi=16384, n_choices=(8
         centers.append((1.1 * math.cos(a), float(rng.uniform(0.0, 1.0)), 1.1 * math.sin(a)))
     return Workload("batched-256", dt=1.0 / 60.0, steps=240, damping=0.9, friction=0.05, hair_hair=True,
                     sphere_radius=0.3, grooms=grooms, wind_const=winds, sphere_centers=centers)
+
+
+def icosphere(subdiv, scale=(1.0, 1.0, 1.0)):
+    """Unit icosphere with `subdiv` midpoint subdivisions (20 * 4^subdiv
+    triangles, outward winding), scaled per axis.  Returns (verts (V,3) f32,
+    tris (T,3) u32)."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(p, dtype=np.float64) / np.linalg.norm(p) for p in v]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5),
+             (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdiv):
+        mid = {}
+
+        def midpoint(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in mid:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                mid[key] = len(verts) - 1
+            return mid[key]
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = midpoint(a, b), midpoint(b, c), midpoint(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    V = (np.array(verts) * np.asarray(scale, dtype=np.float64)).astype(np.float32)
+    return V, np.array(faces, dtype=np.uint32)
 
 
 def sphere_collider(center):

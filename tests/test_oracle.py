@@ -217,6 +217,94 @@ def test_force_ellipsoid_axis_scaled_and_average():
     assert np.allclose(f, [1900.0 * 1.5 / 2, -8.0, 0.0], rtol=1e-12)
 
 
+# ------------------------------------------------------- mesh collider --
+def _np_point_tri_dist(x, a, b, c):
+    """Distance from x to every triangle (a, b, c: (T, 3)), float64, computed
+    independently of the oracle: plane distance when the projection falls
+    inside the triangle (barycentric test), else the nearest of the three
+    edge segments."""
+    n = np.cross(b - a, c - a)
+    n /= np.linalg.norm(n, axis=1)[:, None]
+    dp = np.einsum("ij,ij->i", x - a, n)
+    proj = x - dp[:, None] * n
+    v0, v1, v2 = b - a, c - a, proj - a
+    d00, d01, d11 = (v0 * v0).sum(1), (v0 * v1).sum(1), (v1 * v1).sum(1)
+    d20, d21 = (v2 * v0).sum(1), (v2 * v1).sum(1)
+    den = d00 * d11 - d01 * d01
+    v = (d11 * d20 - d01 * d21) / den
+    w = (d00 * d21 - d01 * d20) / den
+    inside = (v >= 0) & (w >= 0) & (v + w <= 1)
+
+    def seg(p0, p1):
+        d = p1 - p0
+        t = np.clip(np.einsum("ij,ij->i", x - p0, d) / (d * d).sum(1), 0.0, 1.0)
+        return np.linalg.norm(x - (p0 + t[:, None] * d), axis=1)
+    edge = np.minimum(np.minimum(seg(a, b), seg(b, c)), seg(c, a))
+    return np.where(inside, np.abs(dp), edge)
+
+
+def _mesh_params(V, T, margin):
+    return oracle.Params(dt=1 / 64, gravity=(0.0, -8.0, 0.0), spring_k=1900.0, colliders=None,
+                         mesh=oracle.mesh_prepare(V, T), mesh_margin=margin)
+
+
+def test_mesh_face_centre_closed_form():
+    """Just inside a face centre of a convex mesh: F = g + k d n_face, d the depth; just outside: F = g."""
+    V, T = scenes.icosphere(2, (1.0, 1.2, 1.0))
+    p = _mesh_params(V, T, 0.1)
+    g = np.array([0.0, -8.0, 0.0])
+    V64 = V.astype(np.float64)
+    for f in range(0, len(T), 7):
+        a, b, c = V64[T[f]]
+        cen = (a + b + c) / 3
+        n = np.cross(b - a, c - a)
+        n /= np.linalg.norm(n)
+        for depth in (0.003, 0.02):
+            hits, F = oracle.force64(p, cen - depth * n, 0.0)
+            assert hits == 1, (f, depth)
+            assert np.allclose(F, g + 1900.0 * depth * n, rtol=1e-6, atol=1e-6 * 1900)
+        hits, F = oracle.force64(p, cen + 0.01 * n, 0.0)
+        assert hits == 0 and np.array_equal(F, g)
+        hits, F = oracle.force64(p, cen - 0.15 * n, 0.0)  # deeper than the margin
+        assert hits == 0
+
+
+def test_mesh_contact_matches_brute_force_distance_and_convex_inside():
+    """Random points near a scaled icosphere: contact iff inside (all face planes)
+    and within the margin; depth = min point-triangle distance (independent numpy)."""
+    V, T = scenes.icosphere(2, (1.0, 1.2, 1.0))
+    margin = 0.08
+    p = _mesh_params(V, T, margin)
+    V64 = V.astype(np.float64)
+    a, b, c = V64[T[:, 0]], V64[T[:, 1]], V64[T[:, 2]]
+    n = np.cross(b - a, c - a)
+    n /= np.linalg.norm(n, axis=1)[:, None]
+    rng = np.random.default_rng(11)
+    g = np.array([0.0, -8.0, 0.0])
+    seen = {0: 0, 1: 0}
+    for _ in range(400):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        x = d * np.array([1.0, 1.2, 1.0]) * rng.uniform(0.88, 1.06)
+        dist = _np_point_tri_dist(x, a, b, c).min()
+        inside = (np.einsum("ij,ij->i", x - a, n) < 0).all()
+        hits, F = oracle.force64(p, x, 0.0)
+        if abs(dist - margin) < 1e-9 or dist < 1e-9:
+            continue
+        want = 1 if (inside and dist < margin) else 0
+        assert hits == want, (x, dist, inside)
+        seen[want] += 1
+        if hits:
+            assert abs(np.linalg.norm(F - g) - 1900.0 * dist) <= 1e-6 * 1900.0
+    assert seen[0] > 50 and seen[1] > 50
+
+
+def test_mesh_rejects_open_mesh():
+    V, T = scenes.icosphere(1)
+    with pytest.raises(ValueError):
+        oracle.mesh_prepare(V, T[:-1])
+
+
 # ----------------------------------------------------- predict + FTL ----
 def test_single_segment_closed_form():
     """N = 2: the free point moves to the projection of its prediction on the sphere |p - root| = r."""
