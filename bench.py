@@ -288,13 +288,134 @@ def bench_batched(args, hairsim, scenes, wl):
     print(json.dumps(res), flush=True)
 
 
+def bench_served(args, hairsim, scenes):
+    """Served jobs: Poisson arrivals at --rate jobs/s for --seconds; strands
+    log-uniform in [256, 32768], points from {16, 32, 64}, 60-600 steps, wind
+    and sphere keyframes every 0.25 s; 32^3 grids; admission by a 200 GB HBM
+    budget and a per-launch point cap.  Jobs still queued when arrivals end are
+    cancelled; admitted ones run to completion."""
+    import threading
+    rng = np.random.default_rng(scenes.SEED_BASE + 3)
+    # a pool of distinct job shapes; each arrival picks one (synthetic inputs)
+    pool = []
+    for _ in range(48):
+        S = int(round(np.exp(rng.uniform(np.log(256), np.log(32768)))))
+        N = int(rng.choice([16, 32, 64]))
+        roots, nrm = scenes.hemisphere_roots(rng, S)
+        aos, seg = scenes.straight_strands(roots, nrm, N, rng.uniform(0.8, 1.2, S))
+        pool.append((aos, N, seg))
+    G = 32
+    grid = dict(grid_dim=G, grid_origin=(-2.2, -2.2, -2.2), grid_h=4.4 / G)
+    prm = hairsim.step_params(damping=0.9, sphere_radius=0.3, friction=0.05, num_colliders=1, hair_hair=True)
+    budget = 200 << 30
+    srv = hairsim.Server(0, hbm_budget=budget, max_points_per_step=args.max_points, use_graphs=False)
+    n_jobs = int(args.rate * args.seconds * 1.5) + 16
+    gaps = rng.exponential(1.0 / args.rate, n_jobs)
+    arrivals = np.cumsum(gaps)
+    arrivals = arrivals[arrivals < args.seconds]
+    picks = rng.integers(0, len(pool), len(arrivals))
+    steps = rng.integers(60, 601, len(arrivals))
+    key_sets = []
+    for j in range(64):
+        nk = int(np.ceil(600 / 60 / 0.25)) + 2
+        k = np.zeros((nk, 7), dtype=np.float32)
+        k[:, 0] = np.arange(nk) * 0.25
+        k[:, 1:4] = rng.normal(0.0, 1.5, (nk, 3))
+        ang = rng.uniform(0, 2 * np.pi) + np.arange(nk) * 0.4
+        k[:, 4] = 1.1 * np.cos(ang)
+        k[:, 5] = rng.uniform(0.0, 1.0)
+        k[:, 6] = 1.1 * np.sin(ang)
+        key_sets.append(hairsim.keyframes(k))
+    checked = []
+    t0 = time.perf_counter()
+    for i, (ta, pi, ns) in enumerate(zip(arrivals, picks, steps)):
+        now = time.perf_counter() - t0
+        if ta > now:
+            time.sleep(ta - now)
+        aos, N, seg = pool[pi]
+        keep = i % 500 == 0
+        jid, res = srv.submit(aos, N, int(ns), 1.0 / 60.0, key_sets[i % len(key_sets)], prm, seg_length=seg,
+                              keep_result=keep, **grid)
+        if keep:
+            checked.append((jid, res))
+    t_arrivals = time.perf_counter() - t0
+    cancelled = srv.cancel_pending()
+    srv.drain()
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    for jid, res in checked:
+        try:
+            out = srv.wait(jid, 0.0)
+            if not np.isfinite(out).all():
+                print("bench.py: non-finite served result", file=sys.stderr)
+                sys.exit(1)
+        except hairsim.HairSimError:
+            pass  # cancelled before admission
+    srv.close()
+
+    # the plain form on the same shapes: one hs_step launch pair per groom
+    ctx = hairsim.Context(0)
+    import ctypes as C
+    lib = hairsim.lib()
+    plain_pts = 0
+    plain_t = 0.0
+    for aos, N, seg in pool[:16]:
+        gr = ctx.groom(aos, N, seg_length=seg, **grid)
+        t = hairsim.HsTime(1.0 / 60.0, 1.0 / 60.0)
+        cl = (hairsim.HsCollider * 1)(hairsim.HsCollider.from_buffer_copy(
+            scenes.sphere_collider((1.1, 0.5, 0.0)).tobytes()))
+        for _ in range(3):
+            ctx._check(lib.hs_step(ctx._h, gr.id, C.byref(t), cl, 1, C.byref(prm)))
+        ctx.sync()
+        a = time.perf_counter()
+        for _ in range(20):
+            ctx._check(lib.hs_step(ctx._h, gr.id, C.byref(t), cl, 1, C.byref(prm)))
+        ctx.sync()
+        plain_t += time.perf_counter() - a
+        plain_pts += 20 * gr.S * N
+        gr.destroy()
+    ctx.close()
+    plain = plain_pts / plain_t
+    value = st["point_updates"] / wall
+    res = {
+        "metric": "strand-point updates/sec",
+        "value": value,
+        "unit": "point-updates/s",
+        "n_gpus": 1,
+        "steps": int(st["launches"]),
+        "warmup": 0,
+        "ms_per_step": st["p50_step_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"served: Poisson {args.rate:g} jobs/s for {args.seconds:g} s, S log-uniform "
+                               f"[256, 32768], N in {{16,32,64}}, 60-600 steps, keyframes every 0.25 s, 32^3 grids, "
+                               f"200 GB budget, <= {args.max_points} points per launch",
+                   "jobs_submitted": st["submitted"], "jobs_completed": st["completed"],
+                   "jobs_cancelled_unadmitted": cancelled},
+        "served": {"p50_step_ms": st["p50_step_ms"], "p99_step_ms": st["p99_step_ms"],
+                   "max_step_ms": st["max_step_ms"], "p50_wait_ms": st["p50_wait_ms"],
+                   "p99_wait_ms": st["p99_wait_ms"], "launches": st["launches"], "job_steps": st["job_steps"],
+                   "peak_jobs_per_launch": st["peak_jobs_per_launch"],
+                   "peak_resident_gb": st["peak_resident_bytes"] / 1e9, "busy_s": st["busy_s"],
+                   "wall_s": wall, "arrival_window_s": t_arrivals},
+        "per_groom_launch": {"value": plain, "speedup_of_served": value / plain},
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=240)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="plain-100k",
-                    choices=["plain-100k", "large-1m", "plain-1k", "batched-256"])
+                    choices=["plain-100k", "large-1m", "plain-1k", "batched-256", "served"])
+    ap.add_argument("--rate", type=float, default=2000.0, help="served: job arrivals per second")
+    ap.add_argument("--seconds", type=float, default=10.0, help="served: arrival window")
+    ap.add_argument("--max-points", type=int, default=12_000_000, help="served: points per batched launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="batched: eager launches instead of graphs")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -313,6 +434,9 @@ def main():
 
     if args.workload == "batched-256":
         bench_batched(args, hairsim, scenes, scenes.batched())
+        return
+    if args.workload == "served":
+        bench_served(args, hairsim, scenes)
         return
     wl = {"plain-100k": scenes.plain_100k, "large-1m": scenes.large_1m, "plain-1k": scenes.plain_1k}[args.workload]()
     bench_single(args, hairsim, scenes, wl)

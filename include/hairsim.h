@@ -214,6 +214,21 @@ int hs_set_batch_graphs(hs_ctx* ctx, int on);
 /* hs_step_batch calls so far and the graphs captured for them. */
 int hs_batch_stats(const hs_ctx* ctx, uint64_t* batch_calls, uint64_t* graph_captures);
 
+/* Tile choices of the step kernels (no reference counterpart).  Results are
+ * identical for every choice; only the speed changes. */
+typedef struct hs_tile {
+    uint32_t gather_cells;  /* voxels of the LDS window the friction gather stages, 64..2048 */
+    uint32_t splat_cells;   /* voxels of one LDS splat window, 64..2048                   */
+    uint32_t xcd_order;     /* 1: each XCD takes a contiguous run of workgroups           */
+} hs_tile;
+int hs_set_tile(hs_ctx* ctx, const hs_tile* tile);
+int hs_get_tile(const hs_ctx* ctx, hs_tile* tile);
+/* Times every candidate tile on copies of the batch's grooms (their state is
+ * left untouched), `steps` batched steps each after one untimed step, and
+ * keeps the fastest for later steps.  best / best_ms may be NULL. */
+int hs_autotune_batch(hs_ctx* ctx, const hs_batch_item* items, uint32_t num_items, uint32_t steps,
+                      hs_tile* best, double* best_ms);
+
 /* Read the groom back in the reference Strand layout (48*N B per strand) with
  * the velocities after this step's grid friction, plus the indirect draw
  * record {S, 1, 0, 0} the shader's count at compute.comp:302 produces.
@@ -309,6 +324,7 @@ typedef struct hs_job {
     hs_step_params params;        /* wind and collider 0 come from the keys;    */
                                   /* num_colliders is 0 or 1 (the sphere)        */
     float* result_aos;            /* receives the final state, 48*S*N bytes     */
+                                  /* (NULL: the final state is dropped)          */
 } hs_job;
 /* strands_aos, keys and result_aos are the caller's and must stay valid until
  * the job is reported done (hs_server_wait / hs_server_drain). */

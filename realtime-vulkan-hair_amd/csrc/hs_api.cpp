@@ -334,6 +334,9 @@ int build_args(hs_ctx* c, Groom& g, const hs_time* t, const hs_collider* cols, u
     a.ncol = p->num_colliders;
     a.cols = g.cols;
     if (mesh) a.mesh = mesh->args;
+    a.win_gather = c->tile.gather_cells;
+    a.win_splat = c->tile.splat_cells;
+    a.swz = c->tile.xcd_order;
     if (a.ncol > 0) {
         a.sphere_c[0] = cols[0].transform[12];
         a.sphere_c[1] = cols[0].transform[13];
@@ -423,7 +426,7 @@ hipError_t enqueue_batch(const hs_ctx* c, const std::vector<BatchLaunch>& launch
     for (const BatchLaunch& L : launches) {
         e = hs::launch_step_batch(reinterpret_cast<const hs::StepArgs*>(c->dev_blob + L.args_off),
                                   reinterpret_cast<const uint32_t*>(c->dev_blob + L.wg_off), L.ng, L.total_wg,
-                                  (L.v & 1) != 0, (L.v & 2) != 0, collider_mode(L.v), st);
+                                  (L.v & 1) != 0, (L.v & 2) != 0, collider_mode(L.v), c->tile.xcd_order, st);
         if (e != hipSuccess) return e;
     }
     for (const BatchLaunch& L : launches) {
@@ -906,6 +909,7 @@ int hs_step_batch(hs_ctx* c, const hs_batch_item* items, uint32_t n)
         L.br_off = args_bytes + ell_bytes + L.br_off * sizeof(uint32_t);
     }
     key.push_back(bytes);
+    key.push_back(c->tile.xcd_order);
     int rc = ensure_blob(c, bytes);
     if (rc != HS_OK) return rc;
     const int slot = c->slot_next;
@@ -945,7 +949,8 @@ int hs_step_batch(hs_ctx* c, const hs_batch_item* items, uint32_t n)
         for (const BatchLaunch& L : launches)
             HS_TRY(c, hs::launch_step_batch(reinterpret_cast<const hs::StepArgs*>(c->dev_blob + L.args_off),
                                             reinterpret_cast<const uint32_t*>(c->dev_blob + L.wg_off), L.ng,
-                                            L.total_wg, (L.v & 1) != 0, (L.v & 2) != 0, collider_mode(L.v), c->stream));
+                                            L.total_wg, (L.v & 1) != 0, (L.v & 2) != 0, collider_mode(L.v),
+                                            c->tile.xcd_order, c->stream));
         HS_TRY(c, hipEventRecord(ev->e[1], c->stream));
         for (const BatchLaunch& L : launches)
             if (L.v & 2)
@@ -975,6 +980,108 @@ int hs_set_batch_graphs(hs_ctx* c, int on)
         drop_graphs(c);
     }
     c->graphs = on != 0;
+    return HS_OK;
+}
+
+int hs_set_tile(hs_ctx* c, const hs_tile* t)
+{
+    if (!c) return HS_ERR_INVALID;
+    if (!t) return fail(c, HS_ERR_INVALID, "hs_set_tile: null tile");
+    if (t->gather_cells < 64 || t->gather_cells > hs::kWinCells || t->splat_cells < 64 ||
+        t->splat_cells > hs::kWinCells / HS_VOXEL_COPIES || t->xcd_order > 1)
+        return fail(c, HS_ERR_INVALID, "hs_set_tile: window sizes must be in [64, 2048], xcd_order 0 or 1");
+    c->tile = *t;
+    return HS_OK;
+}
+
+int hs_get_tile(const hs_ctx* c, hs_tile* t)
+{
+    if (!c || !t) return HS_ERR_INVALID;
+    *t = c->tile;
+    return HS_OK;
+}
+
+int hs_autotune_batch(hs_ctx* c, const hs_batch_item* items, uint32_t n, uint32_t steps, hs_tile* best,
+                      double* best_ms)
+{
+    if (!c) return HS_ERR_INVALID;
+    if (!items || n == 0 || steps == 0) return fail(c, HS_ERR_INVALID, "hs_autotune_batch: empty batch");
+    HS_TRY(c, hipSetDevice(c->device));
+    // copies of the grooms: the candidates run on these
+    std::vector<hs_batch_item> tmp(items, items + n);
+    std::vector<uint32_t> clones;
+    auto drop = [&] {
+        (void)hipStreamSynchronize(c->stream);
+        for (uint32_t id : clones) {
+            Groom g;
+            if (ctx_take(c, id, &g)) groom_release(g);
+        }
+    };
+    for (uint32_t k = 0; k < n; ++k) {
+        Groom* src = find(c, items[k].groom_id);
+        if (!src) {
+            drop();
+            return fail(c, HS_ERR_NOT_FOUND, "hs_autotune_batch: unknown groom");
+        }
+        Groom g = *src;
+        g.heap = nullptr;
+        g.block = nullptr;
+        hipError_t e = hipMalloc(&g.block, g.bytes);
+        if (e == hipSuccess) {
+            carve(g, static_cast<uint8_t*>(g.block));
+            e = hipMemcpyAsync(g.block, src->block, g.bytes, hipMemcpyDeviceToDevice, c->stream);
+        }
+        if (e != hipSuccess) {
+            if (g.block) (void)hipFree(g.block);
+            drop();
+            return hip_fail(c, e, "hs_autotune_batch: copy");
+        }
+        const uint32_t id = ctx_adopt(c, std::move(g));
+        clones.push_back(id);
+        tmp[k].groom_id = id;
+    }
+    const hs_tile saved = c->tile;
+    const bool graphs = c->graphs;
+    const bool prof = c->profile;
+    c->profile = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = HS_OK;
+    double best_t = 1e300;
+    hs_tile best_tile = saved;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = HS_ERR_HIP;
+    const uint32_t gcand[] = {512, 1024, 2048};
+    const uint32_t scand[] = {512, 1024, 2048};
+    for (uint32_t gc : gcand)
+        for (uint32_t sc : scand)
+            for (uint32_t xo = 0; xo < 2 && rc == HS_OK; ++xo) {
+                if (gc > hs::kWinCells || sc > hs::kWinCells / HS_VOXEL_COPIES) continue;
+                c->tile = hs_tile{gc, sc, xo};
+                rc = hs_step_batch(c, tmp.data(), n);
+                if (rc != HS_OK) break;
+                if (hipEventRecord(e0, c->stream) != hipSuccess) rc = HS_ERR_HIP;
+                for (uint32_t s = 0; s < steps && rc == HS_OK; ++s) rc = hs_step_batch(c, tmp.data(), n);
+                float ms = 0.0f;
+                if (rc == HS_OK && (hipEventRecord(e1, c->stream) != hipSuccess ||
+                                    hipEventSynchronize(e1) != hipSuccess ||
+                                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                    rc = HS_ERR_HIP;
+                if (rc == HS_OK && ms / steps < best_t) {
+                    best_t = ms / steps;
+                    best_tile = c->tile;
+                }
+            }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    c->profile = prof;
+    c->graphs = graphs;
+    drop();
+    if (rc != HS_OK) {
+        c->tile = saved;
+        return rc;
+    }
+    c->tile = best_tile;
+    if (best) *best = best_tile;
+    if (best_ms) *best_ms = best_t;
     return HS_OK;
 }
 

@@ -132,6 +132,7 @@ struct hs_ctx {
     std::vector<BatchGraph> graph_cache;
     uint64_t batch_calls = 0;
     uint64_t graph_captures = 0;
+    hs_tile tile{hs::kWinCells, hs::kWinCells / HS_VOXEL_COPIES, 1};
 };
 
 namespace hsi {

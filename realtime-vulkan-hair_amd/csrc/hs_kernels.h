@@ -86,6 +86,10 @@ struct StepArgs {
     uint32_t ncol;        // colliders in use (0: none, 1: the sphere, >1: + ellipsoids)
     const Collider* cols; // device copy of all ncol colliders (read for ellipsoids 1..ncol-1)
     MeshArgs mesh;        // triangle-mesh collider (mesh.tris null: none)
+    // tile choices (results do not depend on them)
+    uint32_t win_gather;  // voxel cells of the gather window (<= kWinCells)
+    uint32_t win_splat;   // voxel cells of a splat window (<= kWinCells / HS_VOXEL_COPIES)
+    uint32_t swz;         // 1: XCD-aware workgroup order
 };
 
 struct UnpackGrid {
@@ -99,7 +103,7 @@ hipError_t launch_reduce(const GridWork& gw, uint32_t G, uint32_t parity, hipStr
 // Many grooms per launch: per-groom arguments in device memory; wg_start /
 // brick_start are prefix sums (ngroom + 1 entries) of workgroups / bricks.
 hipError_t launch_step_batch(const StepArgs* args_dev, const uint32_t* wg_start_dev, uint32_t ngroom,
-                             uint32_t total_wg, bool gather, bool splat, int col, hipStream_t st);
+                             uint32_t total_wg, bool gather, bool splat, int col, uint32_t swz, hipStream_t st);
 hipError_t launch_reduce_batch(const StepArgs* args_dev, const uint32_t* brick_start_dev, uint32_t ngroom,
                                uint32_t total_bricks, hipStream_t st);
 hipError_t launch_pack(const float* aos_dev, const uint32_t* perm, uint32_t S, uint32_t N, uint32_t Sp,

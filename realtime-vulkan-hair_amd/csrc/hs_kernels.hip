@@ -657,7 +657,7 @@ __device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
                 box.lo[k] = gb[k];
                 box.hi[k] = gb[3 + k];
             }
-            const Plan pl = plan_box(box, (int)kWinCells);
+            const Plan pl = plan_box(box, (int)min(a.win_gather, kWinCells));
             if (pl.ok && pl.ny == 1 && pl.nz == 1) {
                 gwin = chunk_of(box, pl, 0, 0);
                 gwin_ok = true;
@@ -778,7 +778,7 @@ __device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
                 }
             }
             const Box box = block_box(sbox, red);
-            const Plan pl = plan_box(box, kSplatCells);
+            const Plan pl = plan_box(box, min((int)a.win_splat, kSplatCells));
             uint32_t* ctr = a.gw.counters + a.parity * 4;
             if (!box_is_empty(box) && !pl.ok) {
                 // no window fits: device atomics into the overflow grid
@@ -942,7 +942,7 @@ __device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
 template <bool GATHER, bool SPLAT, int COL>
 __global__ __launch_bounds__(kW) void k_step(StepArgs a)
 {
-    const uint32_t wg = xcd_swizzle(blockIdx.x, a.nwg);
+    const uint32_t wg = a.swz ? xcd_swizzle(blockIdx.x, a.nwg) : blockIdx.x;
     if (wg >= a.nwg) return;
     step_body<GATHER, SPLAT, COL>(a, wg);
 }
@@ -964,10 +964,11 @@ __device__ __forceinline__ uint32_t find_groom(const uint32_t* __restrict__ star
 
 template <bool GATHER, bool SPLAT, int COL>
 __global__ __launch_bounds__(kW) void k_step_batch(const StepArgs* __restrict__ args,
-                                                   const uint32_t* __restrict__ wg_start, uint32_t ngroom)
+                                                   const uint32_t* __restrict__ wg_start, uint32_t ngroom,
+                                                   uint32_t swz)
 {
     const uint32_t total = wg_start[ngroom];
-    const uint32_t gwg = xcd_swizzle(blockIdx.x, total);
+    const uint32_t gwg = swz ? xcd_swizzle(blockIdx.x, total) : blockIdx.x;
     if (gwg >= total) return;
     const uint32_t g = find_groom(wg_start, ngroom, gwg);
     // by-value copy: the arguments then live in SGPRs like a kernel argument
@@ -1242,39 +1243,39 @@ hipError_t launch_step(const StepArgs& a, bool gather, bool splat, hipStream_t s
 
 template <bool G, bool S>
 static void launch_batch_e(const StepArgs* args, const uint32_t* wg_start, uint32_t ngroom, int col,
-                           uint32_t blocks, hipStream_t st)
+                           uint32_t swz, uint32_t blocks, hipStream_t st)
 {
     switch (col) {
     case 0:
-        hipLaunchKernelGGL((k_step_batch<G, S, 0>), dim3(blocks), dim3(kW), 0, st, args, wg_start, ngroom);
+        hipLaunchKernelGGL((k_step_batch<G, S, 0>), dim3(blocks), dim3(kW), 0, st, args, wg_start, ngroom, swz);
         break;
     case kColEll:
-        hipLaunchKernelGGL((k_step_batch<G, S, kColEll>), dim3(blocks), dim3(kW), 0, st, args, wg_start, ngroom);
+        hipLaunchKernelGGL((k_step_batch<G, S, kColEll>), dim3(blocks), dim3(kW), 0, st, args, wg_start, ngroom, swz);
         break;
     case kColMesh:
-        hipLaunchKernelGGL((k_step_batch<G, S, kColMesh>), dim3(blocks), dim3(kW), 0, st, args, wg_start, ngroom);
+        hipLaunchKernelGGL((k_step_batch<G, S, kColMesh>), dim3(blocks), dim3(kW), 0, st, args, wg_start, ngroom, swz);
         break;
     default:
         hipLaunchKernelGGL((k_step_batch<G, S, kColEll | kColMesh>), dim3(blocks), dim3(kW), 0, st, args, wg_start,
-                           ngroom);
+                           ngroom, swz);
         break;
     }
 }
 
 hipError_t launch_step_batch(const StepArgs* args_dev, const uint32_t* wg_start_dev, uint32_t ngroom,
-                             uint32_t total_wg, bool gather, bool splat, int col, hipStream_t st)
+                             uint32_t total_wg, bool gather, bool splat, int col, uint32_t swz, hipStream_t st)
 {
     const uint32_t blocks = ((total_wg + 7) / 8) * 8;
     if (gather) {
         if (splat)
-            launch_batch_e<true, true>(args_dev, wg_start_dev, ngroom, col, blocks, st);
+            launch_batch_e<true, true>(args_dev, wg_start_dev, ngroom, col, swz, blocks, st);
         else
-            launch_batch_e<true, false>(args_dev, wg_start_dev, ngroom, col, blocks, st);
+            launch_batch_e<true, false>(args_dev, wg_start_dev, ngroom, col, swz, blocks, st);
     } else {
         if (splat)
-            launch_batch_e<false, true>(args_dev, wg_start_dev, ngroom, col, blocks, st);
+            launch_batch_e<false, true>(args_dev, wg_start_dev, ngroom, col, swz, blocks, st);
         else
-            launch_batch_e<false, false>(args_dev, wg_start_dev, ngroom, col, blocks, st);
+            launch_batch_e<false, false>(args_dev, wg_start_dev, ngroom, col, swz, blocks, st);
     }
     return hipGetLastError();
 }

@@ -206,7 +206,7 @@ void hs_server::admit_loop()
         std::string why;
         int rc = hsi::groom_build(&j->spec.groom, j->spec.strands_aos, up, &arena, &j->g, &why);
         hipError_t e = hipSuccess;
-        if (rc == HS_OK) {
+        if (rc == HS_OK && j->spec.result_aos) {
             e = arena.alloc(&j->staging, (size_t)j->points * 48);
             if (e != hipSuccess) hsi::groom_release(j->g);
         }
@@ -311,7 +311,7 @@ void hs_server::step_loop()
             for (Job* j : active) {
                 hsi::Groom g;
                 if (hsi::ctx_take(ctx, j->groom_id, &g)) hsi::groom_release(g);
-                arena.release(j->staging);
+                if (j->staging) arena.release(j->staging);
                 resident -= j->bytes;
                 admitted_points -= j->points;
                 in_system -= 1;
@@ -363,12 +363,16 @@ void hs_server::read_loop()
         finished.pop_front();
         lk.unlock();
         // the step loop waited for the last launch: the state is final
-        hipError_t e = hsi::groom_unpack(j->g, static_cast<float*>(j->staging), down);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(j->spec.result_aos, j->staging, (size_t)j->points * 48, hipMemcpyDeviceToHost, down);
-        if (e == hipSuccess) e = hipStreamSynchronize(down);
+        hipError_t e = hipSuccess;
+        if (j->spec.result_aos) {
+            e = hsi::groom_unpack(j->g, static_cast<float*>(j->staging), down);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(j->spec.result_aos, j->staging, (size_t)j->points * 48, hipMemcpyDeviceToHost,
+                                   down);
+            if (e == hipSuccess) e = hipStreamSynchronize(down);
+        }
         hsi::groom_release(j->g);
-        arena.release(j->staging);
+        if (j->staging) arena.release(j->staging);
         j->staging = nullptr;
         lk.lock();
         resident -= j->bytes;
@@ -453,7 +457,7 @@ int hs_server_submit(hs_server* s, const hs_job* job, uint64_t* job_id)
     std::string why;
     int rc = hsi::check_groom_desc(&job->groom, job->strands_aos, &why);
     if (rc != HS_OK) return rc;
-    if (job->steps == 0 || !(job->dt > 0.0f) || !job->keys || job->num_keys == 0 || !job->result_aos ||
+    if (job->steps == 0 || !(job->dt > 0.0f) || !job->keys || job->num_keys == 0 ||
         job->params.num_colliders > 1 || job->params.mesh_id != 0)
         return HS_ERR_INVALID;
     auto j = std::make_unique<Job>();
@@ -461,7 +465,7 @@ int hs_server_submit(hs_server* s, const hs_job* job, uint64_t* job_id)
     j->points = (uint64_t)job->groom.num_strands * job->groom.num_points;
     j->bytes = hs_groom_bytes(job->groom.num_strands, job->groom.num_points, job->groom.grid_dim,
                               job->groom.flags) +
-               hsi::round_up(j->points * 48, 256);
+               (job->result_aos ? hsi::round_up(j->points * 48, 256) : 0);
     if (j->bytes > s->cfg.hbm_budget || j->points > s->cfg.max_points_per_step) return HS_ERR_BUDGET;
     j->t_submit = Clock::now();
     std::lock_guard<std::mutex> lk(s->m);

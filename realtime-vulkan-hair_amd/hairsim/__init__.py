@@ -103,6 +103,10 @@ class HsMeshDesc(C.Structure):
     _fields_ = [("grid_dim", C.c_uint32 * 3), ("margin", C.c_float)]
 
 
+class HsTile(C.Structure):
+    _fields_ = [("gather_cells", C.c_uint32), ("splat_cells", C.c_uint32), ("xcd_order", C.c_uint32)]
+
+
 class HsBatchItem(C.Structure):
     _fields_ = [
         ("groom_id", C.c_uint32),
@@ -168,6 +172,10 @@ _SIGS = {
     "hs_grid_near_limit": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint64)]),
     "hs_set_batch_graphs": (C.c_int, [_P, C.c_int]),
     "hs_batch_stats": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "hs_set_tile": (C.c_int, [_P, C.POINTER(HsTile)]),
+    "hs_get_tile": (C.c_int, [_P, C.POINTER(HsTile)]),
+    "hs_autotune_batch": (C.c_int, [_P, C.POINTER(HsBatchItem), C.c_uint32, C.c_uint32, C.POINTER(HsTile),
+                                    C.POINTER(C.c_double)]),
     "hs_mesh_create": (C.c_int, [_P, C.POINTER(C.c_float), C.c_uint32, C.POINTER(C.c_uint32), C.c_uint32,
                                  C.POINTER(HsMeshDesc), C.POINTER(C.c_uint32)]),
     "hs_mesh_destroy": (C.c_int, [_P, C.c_uint32]),
@@ -336,6 +344,22 @@ class Context:
     def batch(self, grooms, max_colliders=1):
         return Batch(self, grooms, max_colliders)
 
+    def set_tile(self, gather_cells=2048, splat_cells=2048, xcd_order=1):
+        self._check(lib().hs_set_tile(self._h, C.byref(HsTile(gather_cells, splat_cells, xcd_order))))
+
+    def get_tile(self):
+        t = HsTile()
+        self._check(lib().hs_get_tile(self._h, C.byref(t)))
+        return (t.gather_cells, t.splat_cells, t.xcd_order)
+
+    def autotune(self, batch, steps=3):
+        """Time every tile candidate on copies of `batch`'s grooms; keep the fastest.
+        Returns ((gather_cells, splat_cells, xcd_order), ms per batched step)."""
+        t = HsTile()
+        ms = C.c_double()
+        self._check(lib().hs_autotune_batch(self._h, batch._items, batch.n, steps, C.byref(t), C.byref(ms)))
+        return (t.gather_cells, t.splat_cells, t.xcd_order), ms.value
+
     def mesh(self, verts, tris, grid_dim=(16, 16, 16), margin=0.05):
         """Register a closed triangle mesh collider; returns its id for step_params(mesh_id=...)."""
         v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
@@ -363,6 +387,29 @@ class Context:
                      grid_origin=grid_origin, grid_h=grid_h, keep_corrections=keep_corrections)
 
 
+def _struct_dtype(st):
+    """numpy view type of a ctypes Structure (pointers as uint64, nested structs flattened)."""
+    names, formats, offsets = [], [], []
+    for name, ct in st._fields_:
+        off = getattr(st, name).offset
+        if isinstance(ct, type) and issubclass(ct, C.Structure):
+            names.append(name)
+            formats.append(_struct_dtype(ct))
+        elif isinstance(ct, type) and issubclass(ct, C.Array):
+            names.append(name)
+            base = {C.c_float: np.float32, C.c_uint32: np.uint32, C.c_int32: np.int32}[ct._type_]
+            formats.append((base, (ct._length_,)))
+        elif isinstance(ct, type) and issubclass(ct, C._Pointer):
+            names.append(name)
+            formats.append(np.uint64)
+        else:
+            names.append(name)
+            formats.append({C.c_float: np.float32, C.c_uint32: np.uint32, C.c_int32: np.int32,
+                            C.c_int: np.int32, C.c_uint64: np.uint64, C.c_double: np.float64}[ct])
+        offsets.append(off)
+    return np.dtype({"names": names, "formats": formats, "offsets": offsets, "itemsize": C.sizeof(st)})
+
+
 class Batch:
     """A fixed set of grooms stepped together by ``hs_step_batch``.
 
@@ -388,8 +435,8 @@ class Batch:
             it.colliders = C.cast(base + k * max(1, max_colliders) * C.sizeof(HsCollider), C.POINTER(HsCollider))
             it.num_colliders = 0
             it.params = C.pointer(self._params[k])
-        self.items = np.ctypeslib.as_array(self._items)
-        self.params = np.ctypeslib.as_array(self._params)
+        self.items = np.frombuffer(self._items, dtype=_struct_dtype(HsBatchItem))
+        self.params = np.frombuffer(self._params, dtype=_struct_dtype(HsStepParams))
         self.colliders = np.frombuffer(self._cols, dtype=np.float32).reshape(n, max(1, max_colliders), 48)
 
     def set_time(self, dt, total_time):
@@ -514,8 +561,10 @@ class Server:
             raise HairSimError(rc, f"{what}: {(msg.value or b'').decode()}")
 
     def submit(self, strands_aos, num_points, steps, dt, keys, params, *, strand_length=2.5, seg_length=None,
-               wind_noise=None, noise_dir=(1.0, 0.0, 0.0), grid_dim=0, grid_origin=(0.0, 0.0, 0.0), grid_h=1.0):
-        """Queue one job; returns (job_id, result array filled when the job is done)."""
+               wind_noise=None, noise_dir=(1.0, 0.0, 0.0), grid_dim=0, grid_origin=(0.0, 0.0, 0.0), grid_h=1.0,
+               keep_result=True):
+        """Queue one job; returns (job_id, result array filled when the job is done, or None
+        with keep_result=False)."""
         aos = np.ascontiguousarray(strands_aos, dtype=np.float32).reshape(-1)
         S = aos.size // (12 * num_points)
         desc = HsGroomDesc()
@@ -532,7 +581,7 @@ class Server:
         desc.grid_h = grid_h
         desc.flags = 0
         karr = keys if isinstance(keys, C.Array) else keyframes(keys)
-        result = np.empty_like(aos)
+        result = np.empty_like(aos) if keep_result else None
         job = HsJob()
         job.groom = desc
         job.strands_aos = _fptr(aos)
@@ -541,7 +590,7 @@ class Server:
         job.keys = C.cast(karr, C.POINTER(HsKeyframe))
         job.num_keys = len(karr)
         job.params = params
-        job.result_aos = _fptr(result)
+        job.result_aos = _fptr(result) if result is not None else None
         jid = C.c_uint64()
         self._check(lib().hs_server_submit(self._h, C.byref(job), C.byref(jid)), "hs_server_submit")
         self._keep[jid.value] = (aos, seg, noise, karr, result)

@@ -143,3 +143,50 @@ def test_batch_ellipsoids_in_blob(ctx):
             assert_bitwise(gr.read()[0], alone[gi], f"groom {gi} graphs={graphs}", wl.grooms[gi].N)
             gr.destroy()
     ctx.set_batch_graphs(False)
+
+
+def test_autotune_leaves_state_and_results_unchanged(ctx):
+    """hs_autotune_batch runs the candidates on copies: the grooms' state is
+    untouched, and stepping with any tile gives the same bits."""
+    wl = small_batch(n=6, seed=41)
+    grs = [make_gpu_groom(ctx, wl, gi) for gi in range(len(wl.grooms))]
+    ref = [make_gpu_groom(ctx, wl, gi) for gi in range(len(wl.grooms))]
+    b = ctx.batch(grs)
+    b.params["damping"] = wl.damping
+    b.params["sphere_radius"] = wl.sphere_radius
+    b.params["friction"] = wl.friction
+    b.params["hair_hair"] = 1
+    b.set_colliders(np.stack([scenes.sphere_collider(wl.sphere(0, gi)) for gi in range(len(grs))])[:, None])
+    b.set_time(wl.dt, float(wl.time(0)))
+    b.step()  # a pending friction and a non-trivial parity before tuning
+    before = [g.read()[0] for g in grs]
+    tile, ms = ctx.autotune(b, steps=2)
+    assert ms > 0.0 and ctx.get_tile() == tile
+    for gi, g in enumerate(grs):
+        assert_bitwise(g.read()[0], before[gi], f"groom {gi} after autotune", wl.grooms[gi].N)
+    for k in range(1, 4):
+        b.set_time(wl.dt, float(wl.time(k)))
+        b.step()
+    ctx.set_tile(2048, 2048, 1)
+    rb = ctx.batch(ref)
+    rb.params[:] = b.params
+    rb.colliders[:] = b.colliders
+    rb.items["num_colliders"] = b.items["num_colliders"]
+    for k in range(0, 4):
+        rb.set_time(wl.dt, float(wl.time(k)))
+        rb.step()
+    for gi in range(len(grs)):
+        assert_bitwise(grs[gi].read()[0], ref[gi].read()[0], f"tuned vs default groom {gi}", wl.grooms[gi].N)
+        assert np.array_equal(grs[gi].read_grid(), ref[gi].read_grid())
+    for g in grs + ref:
+        g.destroy()
+
+
+@pytest.mark.parametrize("tile", [(512, 512, 0), (1024, 2048, 1), (2048, 512, 0), (64, 64, 1)])
+def test_tile_choices_bitwise(ctx, tile):
+    wl = scenes.plain_100k(S=3000, N=20, G=32)
+    ctx.set_tile(*tile)
+    try:
+        run_parity(ctx, wl, 5)
+    finally:
+        ctx.set_tile(2048, 2048, 1)

@@ -150,3 +150,26 @@ def test_keyframes_eval_piecewise_linear():
     assert hairsim.keyframes_eval(keys, 7.0) == ((3.0, 2.0, 1.0), (20.0, 20.0, 40.0))
     assert hairsim.keyframes_eval(keys, 0.5) == ((3.0, 2.0, 1.0), (20.0, 20.0, 40.0))
     assert hairsim.keyframes_eval(keys[:1], 0.3) == ((1.0, 2.0, 3.0), (10.0, 20.0, 30.0))
+
+
+def test_batch_views_write_the_ctypes_arrays():
+    """hairsim.Batch's numpy views alias the item / parameter / collider arrays
+    hs_step_batch reads (built without a GPU: grooms here are only ids)."""
+    class FakeGroom:
+        def __init__(self, i):
+            self.id = i
+    b = hairsim.Batch(None, [FakeGroom(7), FakeGroom(9)], max_colliders=2)
+    b.set_time(0.125, 2.5)
+    b.params["wind"] = np.array([[1, 2, 3], [4, 5, 6]], dtype=np.float32)
+    b.params["friction"] = 0.25
+    cols = np.arange(2 * 2 * 48, dtype=np.float32).reshape(2, 2, 48)
+    b.set_colliders(cols)
+    for k, gid in enumerate((7, 9)):
+        it = b._items[k]
+        assert it.groom_id == gid
+        assert (it.time.deltaTime, it.time.totalTime) == (0.125, 2.5)
+        assert it.num_colliders == 2
+        assert list(it.params.contents.wind) == [1.0 + 3 * k, 2.0 + 3 * k, 3.0 + 3 * k]
+        assert it.params.contents.friction == 0.25
+        assert it.params.contents.num_colliders == 2
+        assert it.colliders[1].inv[0] == cols[k, 1, 16]
