@@ -250,6 +250,10 @@ __device__ __forceinline__ unsigned char* splat_addr(int4* win, const Win& wd, i
     return reinterpret_cast<unsigned char*>(win) + 16 * (win_index(wd, a, b, c) * kSlots + slot);
 }
 
+#ifndef HS_SPLAT_ROTATE
+#define HS_SPLAT_ROTATE 1
+#endif
+
 // Full 2x2x2 block inside the LDS window.
 __device__ __forceinline__ void splat_block(int4* win, const Win& wd, const Rng& r, f3 q, f3 v, int slot)
 {
@@ -258,6 +262,22 @@ __device__ __forceinline__ void splat_block(int4* win, const Win& wd, const Rng&
     const float wz0 = wgt(q.z, r.z0), wz1 = wgt(q.z, r.z1);
     const int sx = 16 * kSlots, sy = sx * wd.ex, sz = sy * wd.ey;
     unsigned char* b = splat_addr(win, wd, r.x0, r.y0, r.z0, slot);
+#if HS_SPLAT_ROTATE
+    // Neighbouring strands share voxels: if every lane added corner 0 first,
+    // up to ~30 lanes of a wavefront would hit one LDS address per atomic.
+    // Lane l visits the corners starting at l % 8, so lanes of one voxel
+    // block hit 8 different cells per instruction.  Integer adds commute and
+    // each corner's weight is still (wx * wy) * wz: the sums are unchanged.
+    const int rot = (int)(threadIdx.x & 7u);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int k = (i + rot) & 7;
+        const bool dx = (k & 1) != 0, dy = (k & 2) != 0, dz = (k & 4) != 0;
+        const float w = ((dx ? wx1 : wx0) * (dy ? wy1 : wy0)) * (dz ? wz1 : wz0);
+        splat_cell(b + (dx ? sx : 0) + (dy ? sy : 0) + (dz ? sz : 0), w, v);
+    }
+    return;
+#endif
     float wxy = wx0 * wy0;
     splat_cell(b, wxy * wz0, v);
     splat_cell(b + sz, wxy * wz1, v);
