@@ -82,7 +82,9 @@ uint64_t carve(Groom& g, uint8_t* base)
         take(g.gw.near_limit, 8);
         g.gw.pool_cap = (uint32_t)L.pool_cap;
         g.gw.desc_cap = (uint32_t)L.desc_cap;
-        g.gw.nb = bricks_per_axis(g.G);
+        g.gw.nbx = bricks_x(g.G);
+        g.gw.nby = bricks_y(g.G);
+        g.gw.nbz = bricks_z(g.G);
     }
     return off;
 }
@@ -890,8 +892,7 @@ int hs_step_batch(hs_ctx* c, const hs_batch_item* items, uint32_t n)
             order.push_back(k);
             key.push_back(((uint64_t)items[k].groom_id << 8) | args[k].ncol);
             wg.push_back(wg.back() + args[k].nwg);
-            const uint32_t nb = args[k].G ? (args[k].G + hs::kBrick - 1) / hs::kBrick : 0;
-            br.push_back(br.back() + ((v & 2) ? nb * nb * nb : 0));
+            br.push_back(br.back() + ((v & 2) ? (uint32_t)bricks(args[k].G) : 0));
         }
         L.total_wg = wg.back();
         L.total_br = br.back();

@@ -16,8 +16,11 @@ namespace hsi {
 
 inline uint64_t round_up(uint64_t x, uint64_t m) { return (x + m - 1) / m * m; }
 inline uint64_t grid_cells(uint32_t G) { return (uint64_t)G * G * G; }
-inline uint32_t bricks_per_axis(uint32_t G) { return (G + hs::kBrick - 1) / hs::kBrick; }
-inline uint64_t bricks(uint32_t G) { return grid_cells(bricks_per_axis(G)); }
+// bricks of the reduce kernel along x, y, z
+inline uint32_t bricks_x(uint32_t G) { return (G + hs::kBX - 1) / hs::kBX; }
+inline uint32_t bricks_y(uint32_t G) { return (G + hs::kBY - 1) / hs::kBY; }
+inline uint32_t bricks_z(uint32_t G) { return (G + hs::kBZ - 1) / hs::kBZ; }
+inline uint64_t bricks(uint32_t G) { return (uint64_t)bricks_x(G) * bricks_y(G) * bricks_z(G); }
 
 // Where a groom's device block comes from: hipMalloc when no heap is given,
 // else a caller-owned heap (the job server's HBM arena).  release() is only

@@ -25,10 +25,12 @@ namespace hs {
 constexpr uint32_t kW = HS_TILE_STRANDS;         // strands per step workgroup
 constexpr uint32_t kL = HS_TILE_POINTS;          // points per segment of the root-to-tip sweep
 constexpr uint32_t kWinCells = HS_WINDOW_CELLS;  // LDS window of one workgroup, 16 B entries
-constexpr uint32_t kBrick = 8;        // reduce kernel: 8x8x8 voxels per workgroup
+// reduce kernel: bricks of kBX x kBY x kBZ voxels, one wavefront each
+constexpr uint32_t kBX = 8, kBY = 4, kBZ = 2;
+constexpr uint32_t kBricksPerWG = 4;
 constexpr uint32_t kMaxColliders = 8;
 constexpr uint32_t kDescInts = 8;     // partial-box descriptor: lo xyz, ext xyz, pool offset, pad
-constexpr uint32_t kBrickList = 512;  // box indices kept per brick (more: the brick scans all boxes)
+constexpr uint32_t kBrickList = 256;  // box indices kept per brick (more: the brick scans all boxes)
 
 // Device-side grid workspace of one groom (all zeroed at creation).
 struct GridWork {
@@ -43,7 +45,8 @@ struct GridWork {
     uint32_t* brick_list; // [2][nbricks][kBrickList] indices of the boxes overlapping a brick
     uint32_t* brick_nz;   // [nbricks] 1 when the brick's grid cells are not all zero
     unsigned long long* near_limit;  // (cell, step) pairs with a component >= 2^30 in magnitude
-    uint32_t pool_cap, desc_cap, nb;  // nb: bricks per axis
+    uint32_t pool_cap, desc_cap;
+    uint32_t nbx, nby, nbz;  // reduce bricks per axis
 };
 
 // Everything one step launch needs, passed by value as the kernel argument.

@@ -905,14 +905,14 @@ __device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
                         if (fits) {
                             // list the box with every brick it overlaps (one device atomic each, in parallel)
                             const uint32_t di = bcast[2];
-                            const int bx0 = cw.x0 / (int)kBrick, by0 = cw.y0 / (int)kBrick, bz0 = cw.z0 / (int)kBrick;
-                            const int nbx = (cw.x0 + cw.ex - 1) / (int)kBrick - bx0 + 1;
-                            const int nby = (cw.y0 + cw.ey - 1) / (int)kBrick - by0 + 1;
-                            const int nbz = (cw.z0 + cw.ez - 1) / (int)kBrick - bz0 + 1;
-                            const uint32_t nbr = a.gw.nb * a.gw.nb * a.gw.nb;
+                            const int bx0 = cw.x0 / (int)kBX, by0 = cw.y0 / (int)kBY, bz0 = cw.z0 / (int)kBZ;
+                            const int nbx = (cw.x0 + cw.ex - 1) / (int)kBX - bx0 + 1;
+                            const int nby = (cw.y0 + cw.ey - 1) / (int)kBY - by0 + 1;
+                            const int nbz = (cw.z0 + cw.ez - 1) / (int)kBZ - bz0 + 1;
+                            const uint32_t nbr = a.gw.nbx * a.gw.nby * a.gw.nbz;
                             for (int t = threadIdx.x; t < nbx * nby * nbz; t += kW) {
-                                const uint32_t b = ((uint32_t)(bz0 + t / (nbx * nby)) * a.gw.nb +
-                                                    (uint32_t)(by0 + (t / nbx) % nby)) * a.gw.nb +
+                                const uint32_t b = ((uint32_t)(bz0 + t / (nbx * nby)) * a.gw.nby +
+                                                    (uint32_t)(by0 + (t / nbx) % nby)) * a.gw.nbx +
                                                    (uint32_t)(bx0 + t % nbx);
                                 const uint32_t at = atomicAdd(a.gw.brick_cnt + a.parity * nbr + b, 1u);
                                 if (at < kBrickList) a.gw.brick_list[((size_t)a.parity * nbr + b) * kBrickList + at] = di;
@@ -999,158 +999,118 @@ __global__ __launch_bounds__(kW) void k_step_batch(const StepArgs* __restrict__ 
 // ---------------------------------------------------------------------------
 // Reduce: partial boxes -> grid (GridCell ints + float cells); clears the grid.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kReduceThreads = 256;
-constexpr uint32_t kListCap = 4096;
+// One wavefront per brick of kBX x kBY x kBZ = 64 voxels, one voxel per lane:
+// each lane sums its voxel over the boxes listed for the brick in registers
+// (wrapping int32 adds, order independent), kInFlight boxes per round.
+constexpr uint32_t kReduceThreads = 64 * kBricksPerWG;
+constexpr uint32_t kInFlight = 8;
+static_assert(kBX * kBY * kBZ == 64, "one voxel per lane");
 
-// Accumulate the cells of partial box `d` that fall in the brick into the LDS
-// brick accumulator; each lane takes up to kPerLane cells, loads first.
-constexpr int kRCells = kBrick * kBrick * kBrick;
-constexpr int kPerLane = kRCells / 64;
-
-struct BoxLoads {
-    int4 v[kPerLane];
-    int li[kPerLane];
+struct BoxRef {
+    int x0, y0, z0, ex, ey, ez, off;
 };
 
-__device__ __forceinline__ void box_loads(BoxLoads& L, const int* __restrict__ d, const int4* __restrict__ pool,
-                                          int x0, int y0, int z0, int x1, int y1, int z1, int lane)
+__device__ __forceinline__ BoxRef box_ref(const int* __restrict__ d)
 {
-    const int dx = d[0], dy = d[1], dz = d[2], ex = d[3], ey = d[4], off = d[6];
-    const int ix0 = max(dx, x0), iy0 = max(dy, y0), iz0 = max(dz, z0);
-    const int nx = min(dx + ex - 1, x1) - ix0 + 1;
-    const int ny = min(dy + ey - 1, y1) - iy0 + 1;
-    const int nz = min(dz + d[5] - 1, z1) - iz0 + 1;
-    const int ncell = (ex > 0 && nx > 0 && ny > 0 && nz > 0) ? nx * ny * nz : 0;
-#pragma unroll
-    for (int k = 0; k < kPerLane; ++k) {
-        const int c = lane + 64 * k;
-        L.li[k] = -1;
-        L.v[k] = make_int4(0, 0, 0, 0);
-        if (c < ncell) {
-            const int cx = ix0 + c % nx, cy = iy0 + (c / nx) % ny, cz = iz0 + c / (nx * ny);
-            L.v[k] = pool[(uint32_t)off + ((uint32_t)(cz - dz) * (uint32_t)ey + (uint32_t)(cy - dy)) * (uint32_t)ex +
-                          (uint32_t)(cx - dx)];
-            L.li[k] = ((cz - z0) * (int)kBrick + (cy - y0)) * (int)kBrick + (cx - x0);
-        }
-    }
+    return BoxRef{d[0], d[1], d[2], d[3], d[4], d[5], d[6]};
 }
 
-__device__ __forceinline__ void box_adds(const BoxLoads& L, uint32_t (*acc)[4])
+// This lane's cell of box b (a zero int4 when the voxel is outside it).
+__device__ __forceinline__ int4 box_cell(const BoxRef& b, const int4* __restrict__ pool, int cx, int cy, int cz)
 {
-#pragma unroll
-    for (int k = 0; k < kPerLane; ++k) {
-        if (L.li[k] < 0) continue;
-        atomicAdd(&acc[L.li[k]][0], (uint32_t)L.v[k].x);
-        atomicAdd(&acc[L.li[k]][1], (uint32_t)L.v[k].y);
-        atomicAdd(&acc[L.li[k]][2], (uint32_t)L.v[k].z);
-        atomicAdd(&acc[L.li[k]][3], (uint32_t)L.v[k].w);
-    }
+    const uint32_t ux = (uint32_t)(cx - b.x0), uy = (uint32_t)(cy - b.y0), uz = (uint32_t)(cz - b.z0);
+    if (ux < (uint32_t)b.ex && uy < (uint32_t)b.ey && uz < (uint32_t)b.ez)
+        return pool[(uint32_t)b.off + (uz * (uint32_t)b.ey + uy) * (uint32_t)b.ex + ux];
+    return make_int4(0, 0, 0, 0);
 }
 
-__device__ __forceinline__ void reduce_body(const GridWork& gw, uint32_t G, uint32_t parity, uint32_t brick,
-                                            bool reset_counters)
+__device__ __forceinline__ void add4(uint4& acc, int4 v)
 {
-    constexpr int kRWaves = kReduceThreads / 64;
-    __shared__ int list[kListCap];
-    __shared__ uint32_t nlist;
-    __shared__ uint32_t acc[kRCells][4];
-    __shared__ uint32_t any_nz;
-    const uint32_t nb = gw.nb;
-    const uint32_t nbr = nb * nb * nb;
-    const uint32_t bx = brick % nb, by = (brick / nb) % nb, bz = brick / (nb * nb);
-    const int x0 = bx * kBrick, y0 = by * kBrick, z0 = bz * kBrick;
-    const int x1 = min(x0 + (int)kBrick, (int)G) - 1, y1 = min(y0 + (int)kBrick, (int)G) - 1,
-              z1 = min(z0 + (int)kBrick, (int)G) - 1;
+    acc.x += (uint32_t)v.x;
+    acc.y += (uint32_t)v.y;
+    acc.z += (uint32_t)v.z;
+    acc.w += (uint32_t)v.w;
+}
+
+__device__ __forceinline__ bool near_limit(int v) { return v >= (1 << 30) || v <= -(1 << 30); }
+
+__device__ __forceinline__ void reduce_brick(const GridWork& gw, uint32_t G, uint32_t parity, uint32_t brick,
+                                             bool reset_counters)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t nbr = gw.nbx * gw.nby * gw.nbz;
+    const uint32_t bx = brick % gw.nbx, by = (brick / gw.nbx) % gw.nby, bz = brick / (gw.nbx * gw.nby);
+    const int cx = (int)(bx * kBX + (lane % kBX)), cy = (int)(by * kBY + (lane / kBX) % kBY),
+              cz = (int)(bz * kBZ + lane / (kBX * kBY));
+    const bool inside = cx < (int)G && cy < (int)G && cz < (int)G;
     const uint32_t* ctr = gw.counters + parity * 4;
-    const uint32_t ndesc = min(ctr[1], gw.desc_cap);
-    const int* desc = gw.desc + (size_t)parity * gw.desc_cap * kDescInts;
-    const int4* pool = gw.pool + (size_t)parity * gw.pool_cap;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const bool ovf = ctr[2] != 0;
     const uint32_t nlisted = gw.brick_cnt[parity * nbr + brick];
-    const uint32_t* blist = gw.brick_list + ((size_t)parity * nbr + brick) * kBrickList;
-
-    if (threadIdx.x == 0) gw.brick_cnt[(parity ^ 1u) * nbr + brick] = 0;  // for the step after next
-    if (reset_counters && threadIdx.x == 0) {
-        uint32_t* nxt = gw.counters + (parity ^ 1u) * 4;
-        nxt[0] = 0;
-        nxt[1] = 0;
-        nxt[2] = 0;
+    if (lane == 0) {
+        gw.brick_cnt[(parity ^ 1u) * nbr + brick] = 0;  // for the step after next
+        if (reset_counters) {
+            uint32_t* nxt = gw.counters + (parity ^ 1u) * 4;
+            nxt[0] = 0;
+            nxt[1] = 0;
+            nxt[2] = 0;
+        }
     }
     // nothing listed, no overflow, and zero already: the brick stays as it is
     if (nlisted == 0 && !ovf && gw.brick_nz[brick] == 0) return;
 
-    for (int c = threadIdx.x; c < kRCells; c += kReduceThreads) {
-        acc[c][0] = 0;
-        acc[c][1] = 0;
-        acc[c][2] = 0;
-        acc[c][3] = 0;
-    }
-    if (threadIdx.x == 0) any_nz = 0;
+    const int* desc = gw.desc + (size_t)parity * gw.desc_cap * kDescInts;
+    const int4* pool = gw.pool + (size_t)parity * gw.pool_cap;
+    uint4 acc = make_uint4(0, 0, 0, 0);
     if (nlisted <= kBrickList) {
-        __syncthreads();
-        // each wavefront takes two boxes at a time: 2 x kPerLane loads in flight per lane
-        for (uint32_t j = 2 * wave; j < nlisted; j += 2 * kRWaves) {
-            BoxLoads A, B;
-            box_loads(A, desc + (size_t)blist[j] * kDescInts, pool, x0, y0, z0, x1, y1, z1, lane);
-            const bool two = j + 1 < nlisted;
-            if (two) box_loads(B, desc + (size_t)blist[j + 1] * kDescInts, pool, x0, y0, z0, x1, y1, z1, lane);
-            box_adds(A, acc);
-            if (two) box_adds(B, acc);
+        const uint32_t* bl = gw.brick_list + ((size_t)parity * nbr + brick) * kBrickList;
+        uint32_t j = 0;
+        for (; j + kInFlight <= nlisted; j += kInFlight) {
+            int4 v[kInFlight];
+#pragma unroll
+            for (uint32_t u = 0; u < kInFlight; ++u)
+                v[u] = box_cell(box_ref(desc + (size_t)bl[j + u] * kDescInts), pool, cx, cy, cz);
+#pragma unroll
+            for (uint32_t u = 0; u < kInFlight; ++u) add4(acc, v[u]);
         }
+        for (; j < nlisted; ++j) add4(acc, box_cell(box_ref(desc + (size_t)bl[j] * kDescInts), pool, cx, cy, cz));
     } else {
         // the brick's list overflowed: scan every box of the step
-        for (uint32_t base = 0; base < ndesc; base += kListCap) {
-            if (threadIdx.x == 0) nlist = 0;
-            __syncthreads();
-            const uint32_t end = min(base + kListCap, ndesc);
-            for (uint32_t i = base + threadIdx.x; i < end; i += kReduceThreads) {
-                const int* d = desc + (size_t)i * kDescInts;
-                const int dx1 = d[0] + d[3] - 1, dy1 = d[1] + d[4] - 1, dz1 = d[2] + d[5] - 1;
-                if (d[3] > 0 && d[0] <= x1 && dx1 >= x0 && d[1] <= y1 && dy1 >= y0 && d[2] <= z1 && dz1 >= z0)
-                    list[atomicAdd(&nlist, 1u)] = (int)i;
-            }
-            __syncthreads();
-            const uint32_t n = nlist;
-            for (uint32_t j = wave; j < n; j += kRWaves) {
-                BoxLoads A;
-                box_loads(A, desc + (size_t)list[j] * kDescInts, pool, x0, y0, z0, x1, y1, z1, lane);
-                box_adds(A, acc);
-            }
-            __syncthreads();
+        const uint32_t ndesc = min(ctr[1], gw.desc_cap);
+        const int x0 = (int)(bx * kBX), y0 = (int)(by * kBY), z0 = (int)(bz * kBZ);
+        for (uint32_t i = 0; i < ndesc; ++i) {
+            const BoxRef b = box_ref(desc + (size_t)i * kDescInts);
+            if (b.ex <= 0 || b.x0 > x0 + (int)kBX - 1 || b.x0 + b.ex <= x0 || b.y0 > y0 + (int)kBY - 1 ||
+                b.y0 + b.ey <= y0 || b.z0 > z0 + (int)kBZ - 1 || b.z0 + b.ez <= z0)
+                continue;
+            add4(acc, box_cell(b, pool, cx, cy, cz));
         }
     }
-    __syncthreads();
-    bool nz = false;
-    uint32_t near = 0;
-    for (int c = threadIdx.x; c < kRCells; c += kReduceThreads) {
-        const int cx = x0 + c % kBrick, cy = y0 + (c / kBrick) % kBrick, cz = z0 + c / (kBrick * kBrick);
-        if (cx > x1 || cy > y1 || cz > z1) continue;
+    bool nz = false, near = false;
+    if (inside) {
         const uint32_t gi = (uint32_t)cx + (uint32_t)cy * G + (uint32_t)cz * G * G;
-        int4 v = make_int4((int)acc[c][0], (int)acc[c][1], (int)acc[c][2], (int)acc[c][3]);
         if (ovf) {
-            const int4 o = gw.overflow[gi];
-            v.x = (int)((uint32_t)v.x + (uint32_t)o.x);
-            v.y = (int)((uint32_t)v.y + (uint32_t)o.y);
-            v.z = (int)((uint32_t)v.z + (uint32_t)o.z);
-            v.w = (int)((uint32_t)v.w + (uint32_t)o.w);
+            add4(acc, gw.overflow[gi]);
             gw.overflow[gi] = make_int4(0, 0, 0, 0);
         }
-        nz |= (v.x | v.y | v.z | v.w) != 0;
-        near += (v.x >= (1 << 30) || v.x <= -(1 << 30) || v.y >= (1 << 30) || v.y <= -(1 << 30) ||
-                 v.z >= (1 << 30) || v.z <= -(1 << 30) || v.w >= (1 << 30) || v.w <= -(1 << 30)) ? 1u : 0u;
+        const int4 v = make_int4((int)acc.x, (int)acc.y, (int)acc.z, (int)acc.w);
+        nz = (v.x | v.y | v.z | v.w) != 0;
+        near = near_limit(v.x) || near_limit(v.y) || near_limit(v.z) || near_limit(v.w);
         gw.grid_i[gi] = v;
         gw.grid_f[gi] = make_float4((float)v.x, (float)v.y, (float)v.z, v.w > 0 ? 1.0f / (float)v.w : 0.0f);
     }
-    if (nz) any_nz = 1;
-    if (near) atomicAdd(gw.near_limit, (unsigned long long)near);
-    __syncthreads();
-    if (threadIdx.x == 0) gw.brick_nz[brick] = any_nz;
+    const uint64_t nz_mask = __ballot(nz);
+    const uint64_t near_mask = __ballot(near);
+    if (lane == 0) {
+        gw.brick_nz[brick] = nz_mask != 0 ? 1u : 0u;
+        if (near_mask) atomicAdd(gw.near_limit, (unsigned long long)__popcll(near_mask));
+    }
 }
 
 __global__ __launch_bounds__(kReduceThreads) void k_reduce(GridWork gw, uint32_t G, uint32_t parity)
 {
-    reduce_body(gw, G, parity, blockIdx.x, blockIdx.x == 0);
+    const uint32_t brick = blockIdx.x * kBricksPerWG + (threadIdx.x >> 6);
+    if (brick >= gw.nbx * gw.nby * gw.nbz) return;
+    reduce_brick(gw, G, parity, brick, brick == 0);
 }
 
 // Many grooms: brick_start[g] is the first global brick of groom g.
@@ -1159,10 +1119,11 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce_batch(const StepArgs*
                                                                  uint32_t ngroom)
 {
     const uint32_t total = brick_start[ngroom];
-    if (blockIdx.x >= total) return;
-    const uint32_t g = find_groom(brick_start, ngroom, blockIdx.x);
-    const uint32_t b = blockIdx.x - brick_start[g];
-    reduce_body(args[g].gw, args[g].G, args[g].parity, b, b == 0);
+    const uint32_t gb = blockIdx.x * kBricksPerWG + (threadIdx.x >> 6);
+    if (gb >= total) return;
+    const uint32_t g = find_groom(brick_start, ngroom, gb);
+    const uint32_t b = gb - brick_start[g];
+    reduce_brick(args[g].gw, args[g].G, args[g].parity, b, b == 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1303,15 +1264,17 @@ hipError_t launch_step_batch(const StepArgs* args_dev, const uint32_t* wg_start_
 hipError_t launch_reduce_batch(const StepArgs* args_dev, const uint32_t* brick_start_dev, uint32_t ngroom,
                                uint32_t total_bricks, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_reduce_batch, dim3(total_bricks), dim3(kReduceThreads), 0, st, args_dev, brick_start_dev,
-                       ngroom);
+    const uint32_t blocks = (total_bricks + kBricksPerWG - 1) / kBricksPerWG;
+    hipLaunchKernelGGL(k_reduce_batch, dim3(blocks), dim3(kReduceThreads), 0, st, args_dev, brick_start_dev, ngroom);
     return hipGetLastError();
 }
 
 hipError_t launch_reduce(const GridWork& gw, uint32_t G, uint32_t parity, hipStream_t st)
 {
-    const uint32_t nb = (G + kBrick - 1) / kBrick;
-    hipLaunchKernelGGL(k_reduce, dim3(nb * nb * nb), dim3(kReduceThreads), 0, st, gw, G, parity);
+    (void)G;
+    const uint32_t nbr = gw.nbx * gw.nby * gw.nbz;
+    hipLaunchKernelGGL(k_reduce, dim3((nbr + kBricksPerWG - 1) / kBricksPerWG), dim3(kReduceThreads), 0, st, gw, G,
+                       parity);
     return hipGetLastError();
 }
 
