@@ -204,6 +204,9 @@ def bench_batched(args, hairsim, scenes, wl):
         b.set_time(wl.dt, float(wl.time(k % wl.steps)))
         b.step()
 
+    # tile choice: timed on copies of the grooms (their state is untouched)
+    b.set_time(wl.dt, float(wl.time(0)))
+    tile, tuned_ms = ctx.autotune(b, steps=3) if not args.no_autotune else (ctx.get_tile(), None)
     k = 0
     for _ in range(args.warmup):
         batch_step(k)
@@ -284,6 +287,8 @@ def bench_batched(args, hairsim, scenes, wl):
                      "whole_step_achieved": whole, "whole_step_frac": whole / PEAK_HBM_GBS},
         "per_groom_launch": {"value": plain, "steps": plain_steps, "ms_per_step": el_plain / plain_steps * 1e3,
                              "speedup_of_batched": value / plain},
+        "tile": {"gather_cells": tile[0], "splat_cells": tile[1], "xcd_order": tile[2],
+                 "autotuned_ms_per_step": tuned_ms},
     }
     print(json.dumps(res), flush=True)
 
@@ -418,6 +423,7 @@ def main():
     ap.add_argument("--max-points", type=int, default=12_000_000, help="served: points per batched launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="batched: eager launches instead of graphs")
+    ap.add_argument("--no-autotune", action="store_true", help="batched: keep the default tile")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="run only a plain elementwise kernel over MIB MiB (byte-counter calibration)")
     args = ap.parse_args()

@@ -1003,7 +1003,7 @@ __global__ __launch_bounds__(kW) void k_step_batch(const StepArgs* __restrict__ 
 // each lane sums its voxel over the boxes listed for the brick in registers
 // (wrapping int32 adds, order independent), kInFlight boxes per round.
 constexpr uint32_t kReduceThreads = 64 * kBricksPerWG;
-constexpr uint32_t kInFlight = 8;
+constexpr uint32_t kInFlight = 16;
 static_assert(kBX * kBY * kBZ == 64, "one voxel per lane");
 
 struct BoxRef {
@@ -1063,16 +1063,34 @@ __device__ __forceinline__ void reduce_brick(const GridWork& gw, uint32_t G, uin
     uint4 acc = make_uint4(0, 0, 0, 0);
     if (nlisted <= kBrickList) {
         const uint32_t* bl = gw.brick_list + ((size_t)parity * nbr + brick) * kBrickList;
-        uint32_t j = 0;
-        for (; j + kInFlight <= nlisted; j += kInFlight) {
-            int4 v[kInFlight];
+        for (uint32_t base = 0; base < nlisted; base += 64) {
+            // lane l fetches box base+l's descriptor: one dependent load chain per 64 boxes
+            const uint32_t m = min(64u, nlisted - base);
+            int4 d0 = make_int4(0, 0, 0, 0), d1 = make_int4(0, 0, 0, 0);
+            if (lane < m) {
+                const int4* dp = reinterpret_cast<const int4*>(desc + (size_t)bl[base + lane] * kDescInts);
+                d0 = dp[0];
+                d1 = dp[1];
+            }
+            // then every lane walks the boxes, kInFlight cell loads at a time
+            for (uint32_t j = 0; j < m; j += kInFlight) {
+                int4 v[kInFlight];
 #pragma unroll
-            for (uint32_t u = 0; u < kInFlight; ++u)
-                v[u] = box_cell(box_ref(desc + (size_t)bl[j + u] * kDescInts), pool, cx, cy, cz);
+                for (uint32_t u = 0; u < kInFlight; ++u) {
+                    v[u] = make_int4(0, 0, 0, 0);
+                    if (j + u < m) {
+                        const int src = (int)(j + u);
+                        const BoxRef b{__builtin_amdgcn_readlane(d0.x, src), __builtin_amdgcn_readlane(d0.y, src),
+                                       __builtin_amdgcn_readlane(d0.z, src), __builtin_amdgcn_readlane(d0.w, src),
+                                       __builtin_amdgcn_readlane(d1.x, src), __builtin_amdgcn_readlane(d1.y, src),
+                                       __builtin_amdgcn_readlane(d1.z, src)};
+                        v[u] = box_cell(b, pool, cx, cy, cz);
+                    }
+                }
 #pragma unroll
-            for (uint32_t u = 0; u < kInFlight; ++u) add4(acc, v[u]);
+                for (uint32_t u = 0; u < kInFlight; ++u) add4(acc, v[u]);
+            }
         }
-        for (; j < nlisted; ++j) add4(acc, box_cell(box_ref(desc + (size_t)bl[j] * kDescInts), pool, cx, cy, cz));
     } else {
         // the brick's list overflowed: scan every box of the step
         const uint32_t ndesc = min(ctr[1], gw.desc_cap);
