@@ -1156,6 +1156,28 @@ int hs_read_grid(hs_ctx* c, uint32_t id, hs_grid_cell* cells)
     return HS_OK;
 }
 
+int hs_grid_work(hs_ctx* c, uint32_t id, hs_grid_work_stats* out)
+{
+    if (!c) return HS_ERR_INVALID;
+    if (!out) return fail(c, HS_ERR_INVALID, "hs_grid_work: null output");
+    Groom* g = find(c, id);
+    if (!g) return fail(c, HS_ERR_NOT_FOUND, "hs_grid_work: unknown groom");
+    std::memset(out, 0, sizeof(*out));
+    if (g->G == 0 || !g->pending) return HS_OK;
+    HS_TRY(c, hipSetDevice(c->device));
+    uint32_t ctr[4] = {0, 0, 0, 0};
+    // the last splat wrote the half before the current parity
+    HS_TRY(c, hipMemcpyAsync(ctr, g->gw.counters + (g->parity ^ 1u) * 4, sizeof(ctr), hipMemcpyDeviceToHost,
+                             c->stream));
+    HS_TRY(c, hipStreamSynchronize(c->stream));
+    out->pool_cells = ctr[0];
+    out->pool_capacity = g->gw.pool_cap;
+    out->boxes = ctr[1];
+    out->box_capacity = g->gw.desc_cap;
+    out->fallback = ctr[2];
+    return HS_OK;
+}
+
 int hs_grid_near_limit(hs_ctx* c, uint32_t id, uint64_t* cells)
 {
     if (!c) return HS_ERR_INVALID;
