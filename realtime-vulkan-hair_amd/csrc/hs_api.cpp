@@ -35,10 +35,14 @@ Layout layout_of(uint32_t S, uint32_t N, uint32_t G)
     l.nseg = (N - 1 + hs::kL - 1) / hs::kL;
     l.pool_cap = 0;
     l.desc_cap = 0;
+    l.list_cap = 0;
     if (G) {
         // partial boxes: room for half a cell per point per step, at least 64k cells
         l.pool_cap = std::max<uint64_t>(65536, (uint64_t)S * (N - 1) / 2);
         l.desc_cap = 4ull * l.nwg * l.nseg + 4096;
+        // about one box per workgroup-segment and step; dense regions list many
+        // of them per brick: room for 1/8 of them, 256 to 2048
+        l.list_cap = (uint32_t)std::min<uint64_t>(2048, std::max<uint64_t>(256, (uint64_t)l.nwg * l.nseg / 8));
     }
     return l;
 }
@@ -77,11 +81,12 @@ uint64_t carve(Groom& g, uint8_t* base)
         take(g.gw.counters, 8 * 4);
         take(g.gw.gbox, (uint64_t)L.nwg * L.nseg * 6 * 4);
         take(g.gw.brick_cnt, bricks(g.G) * 2 * 4);
-        take(g.gw.brick_list, bricks(g.G) * 2 * hs::kBrickList * 4);
+        take(g.gw.brick_list, bricks(g.G) * 2 * L.list_cap * 4);
         take(g.gw.brick_nz, bricks(g.G) * 4);  // zero: every brick of the zeroed grid is empty
         take(g.gw.near_limit, 8);
         g.gw.pool_cap = (uint32_t)L.pool_cap;
         g.gw.desc_cap = (uint32_t)L.desc_cap;
+        g.gw.list_cap = L.list_cap;
         g.gw.nbx = bricks_x(g.G);
         g.gw.nby = bricks_y(g.G);
         g.gw.nbz = bricks_z(g.G);
@@ -746,20 +751,24 @@ int hs_mesh_create(hs_ctx* c, const float* v, uint32_t nv, const uint32_t* t, ui
         return fail(c, HS_ERR_INVALID, "hs_mesh_create: mesh is not closed (or an index is out of range)");
     HS_TRY(c, hipSetDevice(c->device));
     const uint64_t tri_bytes = round_up(b.tris.size() * 4, 256);
+    const uint64_t sph_bytes = round_up(b.spheres.size() * 4, 256);
     const uint64_t start_bytes = round_up(b.cell_start.size() * 4, 256);
     const uint64_t ids_bytes = round_up(std::max<size_t>(b.cell_tris.size(), 1) * 4, 256);
     hsi::Mesh m;
-    m.bytes = tri_bytes + start_bytes + ids_bytes;
+    m.bytes = tri_bytes + sph_bytes + start_bytes + ids_bytes;
     m.ntris = nt;
     HS_TRY(c, hipMalloc(&m.block, m.bytes));
     uint8_t* base = static_cast<uint8_t*>(m.block);
+    uint8_t* sph = base + tri_bytes;
+    uint8_t* starts = sph + sph_bytes;
+    uint8_t* ids = starts + start_bytes;
     hipError_t e = hipMemcpyAsync(base, b.tris.data(), b.tris.size() * 4, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(base + tri_bytes, b.cell_start.data(), b.cell_start.size() * 4, hipMemcpyHostToDevice,
-                           c->stream);
+        e = hipMemcpyAsync(sph, b.spheres.data(), b.spheres.size() * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(starts, b.cell_start.data(), b.cell_start.size() * 4, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess && !b.cell_tris.empty())
-        e = hipMemcpyAsync(base + tri_bytes + start_bytes, b.cell_tris.data(), b.cell_tris.size() * 4,
-                           hipMemcpyHostToDevice, c->stream);
+        e = hipMemcpyAsync(ids, b.cell_tris.data(), b.cell_tris.size() * 4, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         (void)hipFree(m.block);
@@ -767,8 +776,9 @@ int hs_mesh_create(hs_ctx* c, const float* v, uint32_t nv, const uint32_t* t, ui
     }
     hs::MeshArgs& a = m.args;
     a.tris = reinterpret_cast<const float4*>(base);
-    a.cell_start = reinterpret_cast<const uint32_t*>(base + tri_bytes);
-    a.cell_tris = reinterpret_cast<const uint32_t*>(base + tri_bytes + start_bytes);
+    a.spheres = reinterpret_cast<const float4*>(sph);
+    a.cell_start = reinterpret_cast<const uint32_t*>(starts);
+    a.cell_tris = reinterpret_cast<const uint32_t*>(ids);
     a.ox = b.origin[0];
     a.oy = b.origin[1];
     a.oz = b.origin[2];

@@ -30,7 +30,6 @@ constexpr uint32_t kBX = 8, kBY = 4, kBZ = 2;
 constexpr uint32_t kBricksPerWG = 4;
 constexpr uint32_t kMaxColliders = 8;
 constexpr uint32_t kDescInts = 8;     // partial-box descriptor: lo xyz, ext xyz, pool offset, pad
-constexpr uint32_t kBrickList = 256;  // box indices kept per brick (more: the brick scans all boxes)
 
 // Device-side grid workspace of one groom (all zeroed at creation).
 struct GridWork {
@@ -41,12 +40,13 @@ struct GridWork {
     uint32_t* counters;   // [2][4]: cells used, boxes used, overflow flag, pad
     int4* overflow;       // G^3 int4 atomic fallback when the pool is full (normally zero)
     int* gbox;            // [nwg][nseg][6] voxel box each workgroup-segment touched last step
-    uint32_t* brick_cnt;  // [2][nbricks] boxes listed per 8^3 brick (may exceed kBrickList)
-    uint32_t* brick_list; // [2][nbricks][kBrickList] indices of the boxes overlapping a brick
+    uint32_t* brick_cnt;  // [2][nbricks] boxes listed per brick (may exceed list_cap)
+    uint32_t* brick_list; // [2][nbricks][list_cap] indices of the boxes overlapping a brick
     uint32_t* brick_nz;   // [nbricks] 1 when the brick's grid cells are not all zero
     unsigned long long* near_limit;  // (cell, step) pairs with a component >= 2^30 in magnitude
     uint32_t pool_cap, desc_cap;
     uint32_t nbx, nby, nbz;  // reduce bricks per axis
+    uint32_t list_cap;       // box indices kept per brick (more: the brick scans every box)
 };
 
 // Everything one step launch needs, passed by value as the kernel argument.
@@ -55,6 +55,7 @@ struct GridWork {
 // uniform grid of ascending triangle-id lists.
 struct MeshArgs {
     const float4* tris;
+    const float4* spheres;       // per triangle: centroid, bounding radius
     const uint32_t* cell_start;  // [cells + 1]
     const uint32_t* cell_tris;
     float ox, oy, oz;            // grid origin

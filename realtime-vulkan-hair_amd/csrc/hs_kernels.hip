@@ -396,11 +396,23 @@ __device__ __noinline__ float4 mesh_contact(MeshArgs M, float px, float py, floa
     const uint32_t cell = (cz * M.dy + cy) * M.dx + cx;
     const uint32_t i0 = M.cell_start[cell], i1 = M.cell_start[cell + 1];
     float best = M.margin2;
+    // reject radius: a triangle whose bounding sphere lies farther than
+    // sqrt(best) plus a slack (1e-5 (1 + |x|) absolute and 1e-4 relative on
+    // the squares, far above the float rounding of a point-triangle distance
+    // at these magnitudes) cannot be strictly nearer than `best`
+    const float slack = 1e-5f * (1.0f + fmaxf(fmaxf(fabsf(x.x), fabsf(x.y)), fabsf(x.z)));
+    float reach = sqrtf(best) + slack;
     int best_feat = 0;
     uint32_t best_t = 0xffffffffu;
     f3 best_q = mk(0.0f, 0.0f, 0.0f);
+    uint32_t t_next = i0 < i1 ? M.cell_tris[i0] : 0u;
     for (uint32_t i = i0; i < i1; ++i) {
-        const uint32_t t = M.cell_tris[i];
+        const uint32_t t = t_next;
+        if (i + 1 < i1) t_next = M.cell_tris[i + 1];
+        const float4 sp = M.spheres[t];
+        const f3 dc = sub(x, mk(sp.x, sp.y, sp.z));
+        const float lim = sp.w + reach;
+        if (dot(dc, dc) > lim * lim * 1.0001f + 1e-12f) continue;
         const float4* r = M.tris + (size_t)t * 10;
         int feat;
         const f3 q = closest_on_tri(x, ld_f3(r), ld_f3(r + 1), ld_f3(r + 2), &feat);
@@ -408,6 +420,7 @@ __device__ __noinline__ float4 mesh_contact(MeshArgs M, float px, float py, floa
         const float d2 = dot(dq, dq);
         if (d2 < best) {
             best = d2;
+            reach = sqrtf(best) + slack;
             best_t = t;
             best_feat = feat;
             best_q = q;
@@ -915,7 +928,7 @@ __device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
                                                     (uint32_t)(by0 + (t / nbx) % nby)) * a.gw.nbx +
                                                    (uint32_t)(bx0 + t % nbx);
                                 const uint32_t at = atomicAdd(a.gw.brick_cnt + a.parity * nbr + b, 1u);
-                                if (at < kBrickList) a.gw.brick_list[((size_t)a.parity * nbr + b) * kBrickList + at] = di;
+                                if (at < a.gw.list_cap) a.gw.brick_list[((size_t)a.parity * nbr + b) * a.gw.list_cap + at] = di;
                             }
                         }
                         int4* dst = a.gw.pool + (size_t)a.parity * a.gw.pool_cap + off;
@@ -1061,8 +1074,8 @@ __device__ __forceinline__ void reduce_brick(const GridWork& gw, uint32_t G, uin
     const int* desc = gw.desc + (size_t)parity * gw.desc_cap * kDescInts;
     const int4* pool = gw.pool + (size_t)parity * gw.pool_cap;
     uint4 acc = make_uint4(0, 0, 0, 0);
-    if (nlisted <= kBrickList) {
-        const uint32_t* bl = gw.brick_list + ((size_t)parity * nbr + brick) * kBrickList;
+    if (nlisted <= gw.list_cap) {
+        const uint32_t* bl = gw.brick_list + ((size_t)parity * nbr + brick) * gw.list_cap;
         for (uint32_t base = 0; base < nlisted; base += 64) {
             // lane l fetches box base+l's descriptor: one dependent load chain per 64 boxes
             const uint32_t m = min(64u, nlisted - base);

@@ -23,6 +23,7 @@ constexpr int kTriFloats = 40;
 
 struct Built {
     std::vector<float> tris;          // [T][kTriFloats]
+    std::vector<float> spheres;       // [T][4]: centroid, radius (rounded outward) of each triangle
     std::vector<uint32_t> cell_start; // [cells + 1]
     std::vector<uint32_t> cell_tris;  // triangle ids per cell, ascending
     float origin[3] = {0, 0, 0};
@@ -96,6 +97,24 @@ inline bool build(const float* v, uint32_t nv, const uint32_t* t, uint32_t nt, c
             put(4 + k, unit(add(fn[adj[0]], fn[adj[1]])));
             put(7 + k, unit(vn[i[k]]));
         }
+    }
+    // bounding sphere of each triangle about its centroid; the radius is
+    // padded so the float sphere contains the float triangle
+    B.spheres.assign((size_t)nt * 4, 0.0f);
+    for (uint32_t f = 0; f < nt; ++f) {
+        const float* r = &B.tris[(size_t)f * kTriFloats];
+        const D3 a{r[0], r[1], r[2]}, b{r[4], r[5], r[6]}, c{r[8], r[9], r[10]};
+        const D3 cen = mul(add(add(a, b), c), 1.0 / 3.0);
+        const float cf[3] = {(float)cen.x, (float)cen.y, (float)cen.z};
+        const D3 cq{cf[0], cf[1], cf[2]};
+        const double rad = std::max({std::sqrt(dot(sub(a, cq), sub(a, cq))), std::sqrt(dot(sub(b, cq), sub(b, cq))),
+                                     std::sqrt(dot(sub(c, cq), sub(c, cq)))});
+        const double mag = std::max({std::fabs(cq.x), std::fabs(cq.y), std::fabs(cq.z), 1.0});
+        float* sp = &B.spheres[(size_t)f * 4];
+        sp[0] = cf[0];
+        sp[1] = cf[1];
+        sp[2] = cf[2];
+        sp[3] = (float)(rad * (1.0 + 1e-5) + 1e-6 * mag);
     }
     // grid over the mesh bounds grown by the margin
     const double m = margin;
