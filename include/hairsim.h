@@ -246,11 +246,12 @@ int hs_read_grid(hs_ctx* ctx, uint32_t groom_id, hs_grid_cell* cells);
  * of the shader's fixed-point sums (compute.comp:235-251, SCALE 1e6): the
  * format wraps past it, and it is kept as is.  Blocks. */
 int hs_grid_near_limit(hs_ctx* ctx, uint32_t groom_id, uint64_t* cells);
-/* Work of the last splatting step (no reference counterpart): voxel cells
- * of the partial boxes written and the pool's capacity, boxes written and
- * their capacity, and whether anything fell back to device atomics (a box
- * larger than any LDS window, or a full pool).  Zeros before the first such
- * step.  Blocks. */
+/* Work of the last splatting step (no reference counterpart): the voxel
+ * cells and boxes of the shared part of the partial-box pool (each
+ * workgroup-segment's first box goes to a slot of its own and is not
+ * counted) with the shared capacities, and whether anything
+ * fell back to device atomics (a box larger than any LDS window, or a full
+ * pool).  Zeros before the first such step.  Blocks. */
 typedef struct hs_grid_work_stats {
     uint32_t pool_cells, pool_capacity, boxes, box_capacity, fallback;
 } hs_grid_work_stats;

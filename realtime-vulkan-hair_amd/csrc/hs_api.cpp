@@ -38,8 +38,12 @@ Layout layout_of(uint32_t S, uint32_t N, uint32_t G)
     l.list_cap = 0;
     if (G) {
         // partial boxes: room for half a cell per point per step, at least 64k cells
-        l.pool_cap = std::max<uint64_t>(65536, (uint64_t)S * (N - 1) / 2);
-        l.desc_cap = 4ull * l.nwg * l.nseg + 4096;
+        // each workgroup-segment's slot (its first box always fits), then a
+        // shared part for further chunks: half a cell per point per step (at
+        // least 64k cells) and three boxes per segment
+        const uint64_t nws = (uint64_t)l.nwg * l.nseg;
+        l.pool_cap = nws * hs::kSlotCells + std::max<uint64_t>(65536, (uint64_t)S * (N - 1) / 2);
+        l.desc_cap = nws + 3 * nws + 4096;
         // about one box per workgroup-segment and step; dense regions list many
         // of them per brick: room for 1/8 of them, 256 to 2048
         l.list_cap = (uint32_t)std::min<uint64_t>(2048, std::max<uint64_t>(256, (uint64_t)l.nwg * l.nseg / 8));
@@ -86,6 +90,7 @@ uint64_t carve(Groom& g, uint8_t* base)
         take(g.gw.near_limit, 8);
         g.gw.pool_cap = (uint32_t)L.pool_cap;
         g.gw.desc_cap = (uint32_t)L.desc_cap;
+        g.gw.nws = L.nwg * L.nseg;
         g.gw.list_cap = L.list_cap;
         g.gw.nbx = bricks_x(g.G);
         g.gw.nby = bricks_y(g.G);
@@ -754,14 +759,16 @@ int hs_mesh_create(hs_ctx* c, const float* v, uint32_t nv, const uint32_t* t, ui
     const uint64_t sph_bytes = round_up(b.spheres.size() * 4, 256);
     const uint64_t start_bytes = round_up(b.cell_start.size() * 4, 256);
     const uint64_t ids_bytes = round_up(std::max<size_t>(b.cell_tris.size(), 1) * 4, 256);
+    const uint64_t csph_bytes = round_up(std::max<size_t>(b.cell_spheres.size(), 4) * 4, 256);
     hsi::Mesh m;
-    m.bytes = tri_bytes + sph_bytes + start_bytes + ids_bytes;
+    m.bytes = tri_bytes + sph_bytes + start_bytes + ids_bytes + csph_bytes;
     m.ntris = nt;
     HS_TRY(c, hipMalloc(&m.block, m.bytes));
     uint8_t* base = static_cast<uint8_t*>(m.block);
     uint8_t* sph = base + tri_bytes;
     uint8_t* starts = sph + sph_bytes;
     uint8_t* ids = starts + start_bytes;
+    uint8_t* csph = ids + ids_bytes;
     hipError_t e = hipMemcpyAsync(base, b.tris.data(), b.tris.size() * 4, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(sph, b.spheres.data(), b.spheres.size() * 4, hipMemcpyHostToDevice, c->stream);
@@ -769,6 +776,9 @@ int hs_mesh_create(hs_ctx* c, const float* v, uint32_t nv, const uint32_t* t, ui
         e = hipMemcpyAsync(starts, b.cell_start.data(), b.cell_start.size() * 4, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess && !b.cell_tris.empty())
         e = hipMemcpyAsync(ids, b.cell_tris.data(), b.cell_tris.size() * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && !b.cell_spheres.empty())
+        e = hipMemcpyAsync(csph, b.cell_spheres.data(), b.cell_spheres.size() * 4, hipMemcpyHostToDevice,
+                           c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         (void)hipFree(m.block);
@@ -779,6 +789,7 @@ int hs_mesh_create(hs_ctx* c, const float* v, uint32_t nv, const uint32_t* t, ui
     a.spheres = reinterpret_cast<const float4*>(sph);
     a.cell_start = reinterpret_cast<const uint32_t*>(starts);
     a.cell_tris = reinterpret_cast<const uint32_t*>(ids);
+    a.cell_spheres = reinterpret_cast<const float4*>(csph);
     a.ox = b.origin[0];
     a.oy = b.origin[1];
     a.oz = b.origin[2];
@@ -1181,9 +1192,9 @@ int hs_grid_work(hs_ctx* c, uint32_t id, hs_grid_work_stats* out)
                              c->stream));
     HS_TRY(c, hipStreamSynchronize(c->stream));
     out->pool_cells = ctr[0];
-    out->pool_capacity = g->gw.pool_cap;
+    out->pool_capacity = g->gw.pool_cap - g->gw.nws * hs::kSlotCells;
     out->boxes = ctr[1];
-    out->box_capacity = g->gw.desc_cap;
+    out->box_capacity = g->gw.desc_cap - g->gw.nws;
     out->fallback = ctr[2];
     return HS_OK;
 }

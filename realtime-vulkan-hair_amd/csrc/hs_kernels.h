@@ -30,6 +30,8 @@ constexpr uint32_t kBX = 8, kBY = 4, kBZ = 2;
 constexpr uint32_t kBricksPerWG = 4;
 constexpr uint32_t kMaxColliders = 8;
 constexpr uint32_t kDescInts = 8;     // partial-box descriptor: lo xyz, ext xyz, pool offset, pad
+constexpr uint32_t kSlotCells = HS_WINDOW_CELLS / HS_VOXEL_COPIES;  // pool cells a workgroup-segment owns
+                                                                   // for its first box (a whole splat window)
 
 // Device-side grid workspace of one groom (all zeroed at creation).
 struct GridWork {
@@ -37,14 +39,18 @@ struct GridWork {
     float4* grid_f;       // G^3 (float(vx), float(vy), float(vz), density > 0 ? 1/float(density) : 0)
     int4* pool;           // partial boxes written by the step kernel, [2][pool_cap]
     int* desc;            // partial-box descriptors, [2][desc_cap][kDescInts]
-    uint32_t* counters;   // [2][4]: cells used, boxes used, overflow flag, pad
+    uint32_t* counters;   // [2][4]: shared cells used, shared boxes used, fallback flag, pad
     int4* overflow;       // G^3 int4 atomic fallback when the pool is full (normally zero)
     int* gbox;            // [nwg][nseg][6] voxel box each workgroup-segment touched last step
     uint32_t* brick_cnt;  // [2][nbricks] boxes listed per brick (may exceed list_cap)
     uint32_t* brick_list; // [2][nbricks][list_cap] indices of the boxes overlapping a brick
     uint32_t* brick_nz;   // [nbricks] 1 when the brick's grid cells are not all zero
     unsigned long long* near_limit;  // (cell, step) pairs with a component >= 2^30 in magnitude
+    // pool / descriptors per parity: first each workgroup-segment's own slot
+    // (kSlotCells cells, descriptor index = its number), then a shared part
+    // taken with atomics; pool_cap / desc_cap count both
     uint32_t pool_cap, desc_cap;
+    uint32_t nws;            // workgroup-segments: owners of the slots
     uint32_t nbx, nby, nbz;  // reduce bricks per axis
     uint32_t list_cap;       // box indices kept per brick (more: the brick scans every box)
 };
@@ -58,6 +64,7 @@ struct MeshArgs {
     const float4* spheres;       // per triangle: centroid, bounding radius
     const uint32_t* cell_start;  // [cells + 1]
     const uint32_t* cell_tris;
+    const float4* cell_spheres;  // bounding sphere of each cell_tris entry
     float ox, oy, oz;            // grid origin
     float icx, icy, icz;         // 1 / cell size
     uint32_t dx, dy, dz;         // cells per axis

@@ -405,25 +405,35 @@ __device__ __noinline__ float4 mesh_contact(MeshArgs M, float px, float py, floa
     int best_feat = 0;
     uint32_t best_t = 0xffffffffu;
     f3 best_q = mk(0.0f, 0.0f, 0.0f);
-    uint32_t t_next = i0 < i1 ? M.cell_tris[i0] : 0u;
-    for (uint32_t i = i0; i < i1; ++i) {
-        const uint32_t t = t_next;
-        if (i + 1 < i1) t_next = M.cell_tris[i + 1];
-        const float4 sp = M.spheres[t];
-        const f3 dc = sub(x, mk(sp.x, sp.y, sp.z));
-        const float lim = sp.w + reach;
-        if (dot(dc, dc) > lim * lim * 1.0001f + 1e-12f) continue;
-        const float4* r = M.tris + (size_t)t * 10;
-        int feat;
-        const f3 q = closest_on_tri(x, ld_f3(r), ld_f3(r + 1), ld_f3(r + 2), &feat);
-        const f3 dq = sub(x, q);
-        const float d2 = dot(dq, dq);
-        if (d2 < best) {
-            best = d2;
-            reach = sqrtf(best) + slack;
-            best_t = t;
-            best_feat = feat;
-            best_q = q;
+    // candidates four at a time: their ids and spheres are contiguous in the
+    // cell's list, so all eight loads are in flight together
+    for (uint32_t i = i0; i < i1; i += 4) {
+        float4 sp[4];
+        uint32_t tt[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+            const uint32_t e = min(i + u, i1 - 1);
+            sp[u] = M.cell_spheres[e];
+            tt[u] = M.cell_tris[e];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+            if (i + u >= i1) break;
+            const f3 dc = sub(x, mk(sp[u].x, sp[u].y, sp[u].z));
+            const float lim = sp[u].w + reach;
+            if (dot(dc, dc) > lim * lim * 1.0001f + 1e-12f) continue;
+            const float4* r = M.tris + (size_t)tt[u] * 10;
+            int feat;
+            const f3 q = closest_on_tri(x, ld_f3(r), ld_f3(r + 1), ld_f3(r + 2), &feat);
+            const f3 dq = sub(x, q);
+            const float d2 = dot(dq, dq);
+            if (d2 < best) {
+                best = d2;
+                reach = sqrtf(best) + slack;
+                best_t = tt[u];
+                best_feat = feat;
+                best_q = q;
+            }
         }
     }
     if (best_t == 0xffffffffu) return none;
@@ -491,11 +501,24 @@ __device__ __forceinline__ f3 force_at(f3 x, f3 fs, const StepArgs& a)
 // ---------------------------------------------------------------------------
 constexpr int kWaves = kW / 64;
 
+// Minimum over the wavefront (every lane active): DPP steps within each row
+// of 16 lanes (VALU moves, no LDS traffic), then the four row results.
+#ifndef HS_DPP_MIN
+#define HS_DPP_MIN 1
+#endif
 __device__ __forceinline__ int wave_min(int v)
 {
+#if !HS_DPP_MIN
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) v = min(v, __shfl_xor(v, m, 64));
     return v;
+#endif
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false));  // row_mirror
+    return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+               min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
 }
 
 // Box of voxels (inclusive bounds).  Empty: lo = INT_MAX, hi = INT_MIN.
@@ -832,6 +855,9 @@ __device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
                 }
                 if (threadIdx.x == 0) atomicOr(ctr + 2, 1u);
             }
+            // this workgroup-segment's own pool slot and descriptor
+            const uint32_t ws = wg * a.nseg + sg;
+            bool slot_used = false;
             if (pl.ok) {
                 uint32_t done = 0;  // bit kL: carried point
                 for (int iz = 0; iz < pl.nz; ++iz)
@@ -892,9 +918,21 @@ __device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
                             }
                         }
                         __syncthreads();
+#ifndef HS_OWN_SLOTS
+#define HS_OWN_SLOTS 1
+#endif
+                        const bool own = HS_OWN_SLOTS && !slot_used && cells <= (int)kSlotCells;
+                        slot_used |= own;
                         if (threadIdx.x == 0) {
-                            const uint32_t off = atomicAdd(ctr + 0, (uint32_t)cells);
-                            const uint32_t di = atomicAdd(ctr + 1, 1u);
+                            uint32_t off, di;
+                            if (own) {
+                                // no allocator: the slot is this workgroup-segment's
+                                off = ws * kSlotCells;
+                                di = ws;
+                            } else {
+                                off = a.gw.nws * kSlotCells + atomicAdd(ctr + 0, (uint32_t)cells);
+                                di = a.gw.nws + atomicAdd(ctr + 1, 1u);
+                            }
                             const bool fits = (uint64_t)off + cells <= a.gw.pool_cap && di < a.gw.desc_cap;
                             if (di < a.gw.desc_cap) {
                                 int* d = a.gw.desc + ((size_t)a.parity * a.gw.desc_cap + di) * kDescInts;
@@ -960,6 +998,10 @@ __device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
                         }
                         __syncthreads();
                     }
+            }
+            if (!slot_used && threadIdx.x == 0) {
+                // the slot's descriptor from two steps ago must not be read
+                a.gw.desc[((size_t)a.parity * a.gw.desc_cap + ws) * kDescInts + 3] = 0;
             }
         }
         // carry the segment's last point into the next segment
@@ -1106,7 +1148,7 @@ __device__ __forceinline__ void reduce_brick(const GridWork& gw, uint32_t G, uin
         }
     } else {
         // the brick's list overflowed: scan every box of the step
-        const uint32_t ndesc = min(ctr[1], gw.desc_cap);
+        const uint32_t ndesc = min(gw.nws + ctr[1], gw.desc_cap);
         const int x0 = (int)(bx * kBX), y0 = (int)(by * kBY), z0 = (int)(bz * kBZ);
         for (uint32_t i = 0; i < ndesc; ++i) {
             const BoxRef b = box_ref(desc + (size_t)i * kDescInts);

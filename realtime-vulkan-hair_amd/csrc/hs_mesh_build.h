@@ -26,6 +26,7 @@ struct Built {
     std::vector<float> spheres;       // [T][4]: centroid, radius (rounded outward) of each triangle
     std::vector<uint32_t> cell_start; // [cells + 1]
     std::vector<uint32_t> cell_tris;  // triangle ids per cell, ascending
+    std::vector<float> cell_spheres;  // [entries][4]: the bounding sphere of each listed triangle
     float origin[3] = {0, 0, 0};
     float cell[3] = {1, 1, 1};
     float inv_cell[3] = {1, 1, 1};
@@ -154,6 +155,11 @@ inline bool build(const float* v, uint32_t nv, const uint32_t* t, uint32_t nt, c
     B.cell_tris.clear();
     B.cell_tris.reserve(B.cell_start[ncell]);
     for (auto& l : lists) B.cell_tris.insert(B.cell_tris.end(), l.begin(), l.end());
+    // the spheres again, in list order: a cell's candidates are read as one
+    // contiguous stream, without waiting for each id first
+    B.cell_spheres.resize(B.cell_tris.size() * 4);
+    for (size_t e = 0; e < B.cell_tris.size(); ++e)
+        for (int k = 0; k < 4; ++k) B.cell_spheres[4 * e + k] = B.spheres[(size_t)B.cell_tris[e] * 4 + k];
     return true;
 }
 
