@@ -35,7 +35,6 @@ Layout layout_of(uint32_t S, uint32_t N, uint32_t G)
     l.nseg = (N - 1 + hs::kL - 1) / hs::kL;
     l.pool_cap = 0;
     l.desc_cap = 0;
-    l.list_cap = 0;
     if (G) {
         // partial boxes: room for half a cell per point per step, at least 64k cells
         // each workgroup-segment's slot (its first box always fits), then a
@@ -44,9 +43,6 @@ Layout layout_of(uint32_t S, uint32_t N, uint32_t G)
         const uint64_t nws = (uint64_t)l.nwg * l.nseg;
         l.pool_cap = nws * hs::kSlotCells + std::max<uint64_t>(65536, (uint64_t)S * (N - 1) / 2);
         l.desc_cap = nws + 3 * nws + 4096;
-        // about one box per workgroup-segment and step; dense regions list many
-        // of them per brick: room for 1/8 of them, 256 to 2048
-        l.list_cap = (uint32_t)std::min<uint64_t>(2048, std::max<uint64_t>(256, (uint64_t)l.nwg * l.nseg / 8));
     }
     return l;
 }
@@ -85,13 +81,12 @@ uint64_t carve(Groom& g, uint8_t* base)
         take(g.gw.counters, 8 * 4);
         take(g.gw.gbox, (uint64_t)L.nwg * L.nseg * 6 * 4);
         take(g.gw.brick_cnt, bricks(g.G) * 2 * 4);
-        take(g.gw.brick_list, bricks(g.G) * 2 * L.list_cap * 4);
+        take(g.gw.brick_list, bricks(g.G) * 2 * hs::kBrickList * 4);
         take(g.gw.brick_nz, bricks(g.G) * 4);  // zero: every brick of the zeroed grid is empty
         take(g.gw.near_limit, 8);
         g.gw.pool_cap = (uint32_t)L.pool_cap;
         g.gw.desc_cap = (uint32_t)L.desc_cap;
         g.gw.nws = L.nwg * L.nseg;
-        g.gw.list_cap = L.list_cap;
         g.gw.nbx = bricks_x(g.G);
         g.gw.nby = bricks_y(g.G);
         g.gw.nbz = bricks_z(g.G);
@@ -760,8 +755,9 @@ int hs_mesh_create(hs_ctx* c, const float* v, uint32_t nv, const uint32_t* t, ui
     const uint64_t start_bytes = round_up(b.cell_start.size() * 4, 256);
     const uint64_t ids_bytes = round_up(std::max<size_t>(b.cell_tris.size(), 1) * 4, 256);
     const uint64_t csph_bytes = round_up(std::max<size_t>(b.cell_spheres.size(), 4) * 4, 256);
+    const uint64_t key_bytes = round_up(std::max<size_t>(b.cell_keys.size(), 1) * 4, 256);
     hsi::Mesh m;
-    m.bytes = tri_bytes + sph_bytes + start_bytes + ids_bytes + csph_bytes;
+    m.bytes = tri_bytes + sph_bytes + start_bytes + ids_bytes + csph_bytes + key_bytes;
     m.ntris = nt;
     HS_TRY(c, hipMalloc(&m.block, m.bytes));
     uint8_t* base = static_cast<uint8_t*>(m.block);
@@ -769,6 +765,7 @@ int hs_mesh_create(hs_ctx* c, const float* v, uint32_t nv, const uint32_t* t, ui
     uint8_t* starts = sph + sph_bytes;
     uint8_t* ids = starts + start_bytes;
     uint8_t* csph = ids + ids_bytes;
+    uint8_t* keys = csph + csph_bytes;
     hipError_t e = hipMemcpyAsync(base, b.tris.data(), b.tris.size() * 4, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(sph, b.spheres.data(), b.spheres.size() * 4, hipMemcpyHostToDevice, c->stream);
@@ -779,6 +776,8 @@ int hs_mesh_create(hs_ctx* c, const float* v, uint32_t nv, const uint32_t* t, ui
     if (e == hipSuccess && !b.cell_spheres.empty())
         e = hipMemcpyAsync(csph, b.cell_spheres.data(), b.cell_spheres.size() * 4, hipMemcpyHostToDevice,
                            c->stream);
+    if (e == hipSuccess && !b.cell_keys.empty())
+        e = hipMemcpyAsync(keys, b.cell_keys.data(), b.cell_keys.size() * 4, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         (void)hipFree(m.block);
@@ -790,6 +789,8 @@ int hs_mesh_create(hs_ctx* c, const float* v, uint32_t nv, const uint32_t* t, ui
     a.cell_start = reinterpret_cast<const uint32_t*>(starts);
     a.cell_tris = reinterpret_cast<const uint32_t*>(ids);
     a.cell_spheres = reinterpret_cast<const float4*>(csph);
+    a.cell_keys = reinterpret_cast<const float*>(keys);
+    a.half_diag = b.half_diag;
     a.ox = b.origin[0];
     a.oy = b.origin[1];
     a.oz = b.origin[2];

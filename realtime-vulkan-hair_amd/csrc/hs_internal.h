@@ -35,7 +35,6 @@ struct DevHeap {
 struct Layout {
     uint32_t Sp, nwg, nseg;
     uint64_t pool_cap, desc_cap;
-    uint32_t list_cap;  // box indices per reduce brick
 };
 
 Layout layout_of(uint32_t S, uint32_t N, uint32_t G);

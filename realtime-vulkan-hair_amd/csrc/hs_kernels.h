@@ -30,6 +30,7 @@ constexpr uint32_t kBX = 8, kBY = 4, kBZ = 2;
 constexpr uint32_t kBricksPerWG = 4;
 constexpr uint32_t kMaxColliders = 8;
 constexpr uint32_t kDescInts = 8;     // partial-box descriptor: lo xyz, ext xyz, pool offset, pad
+constexpr uint32_t kBrickList = 2048;  // box indices kept per reduce brick (more: it scans every box)
 constexpr uint32_t kSlotCells = HS_WINDOW_CELLS / HS_VOXEL_COPIES;  // pool cells a workgroup-segment owns
                                                                    // for its first box (a whole splat window)
 
@@ -42,8 +43,8 @@ struct GridWork {
     uint32_t* counters;   // [2][4]: shared cells used, shared boxes used, fallback flag, pad
     int4* overflow;       // G^3 int4 atomic fallback when the pool is full (normally zero)
     int* gbox;            // [nwg][nseg][6] voxel box each workgroup-segment touched last step
-    uint32_t* brick_cnt;  // [2][nbricks] boxes listed per brick (may exceed list_cap)
-    uint32_t* brick_list; // [2][nbricks][list_cap] indices of the boxes overlapping a brick
+    uint32_t* brick_cnt;  // [2][nbricks] boxes listed per brick (may exceed kBrickList)
+    uint32_t* brick_list; // [2][nbricks][kBrickList] indices of the boxes overlapping a brick
     uint32_t* brick_nz;   // [nbricks] 1 when the brick's grid cells are not all zero
     unsigned long long* near_limit;  // (cell, step) pairs with a component >= 2^30 in magnitude
     // pool / descriptors per parity: first each workgroup-segment's own slot
@@ -52,7 +53,6 @@ struct GridWork {
     uint32_t pool_cap, desc_cap;
     uint32_t nws;            // workgroup-segments: owners of the slots
     uint32_t nbx, nby, nbz;  // reduce bricks per axis
-    uint32_t list_cap;       // box indices kept per brick (more: the brick scans every box)
 };
 
 // Everything one step launch needs, passed by value as the kernel argument.
@@ -65,6 +65,8 @@ struct MeshArgs {
     const uint32_t* cell_start;  // [cells + 1]
     const uint32_t* cell_tris;
     const float4* cell_spheres;  // bounding sphere of each cell_tris entry
+    const float* cell_keys;      // lower bound of the cell-triangle distance (ascending per cell)
+    float half_diag;             // half the cell diagonal
     float ox, oy, oz;            // grid origin
     float icx, icy, icz;         // 1 / cell size
     uint32_t dx, dy, dz;         // cells per axis

@@ -383,6 +383,8 @@ __device__ __forceinline__ f3 ld_f3(const float4* p)
 // pseudo-normal n of q's feature (Baerentzen & Aanaes, IEEE TVCG 2005).
 // Returns (k |x - q|) normalize(q - x) in xyz and 1 in w on contact, else 0.
 // No reference counterpart.  Not inlined: one copy serves every point.
+constexpr uint32_t kMeshRound = 8;  // mesh candidates loaded per round
+
 __device__ __noinline__ float4 mesh_contact(MeshArgs M, float px, float py, float pz, float k)
 {
     const f3 x = mk(px, py, pz);
@@ -395,6 +397,11 @@ __device__ __noinline__ float4 mesh_contact(MeshArgs M, float px, float py, floa
                    cz = min((uint32_t)gz, M.dz - 1);
     const uint32_t cell = (cz * M.dy + cy) * M.dx + cx;
     const uint32_t i0 = M.cell_start[cell], i1 = M.cell_start[cell + 1];
+    // distance from x to its cell's centre: a candidate's distance is at least
+    // its key minus this (the key bounds the distance from the centre)
+    const f3 cc = mk(M.ox + ((float)cx + 0.5f) / M.icx, M.oy + ((float)cy + 0.5f) / M.icy,
+                     M.oz + ((float)cz + 0.5f) / M.icz);
+    const float dxc = length(sub(x, cc));
     float best = M.margin2;
     // reject radius: a triangle whose bounding sphere lies farther than
     // sqrt(best) plus a slack (1e-5 (1 + |x|) absolute and 1e-4 relative on
@@ -405,20 +412,32 @@ __device__ __noinline__ float4 mesh_contact(MeshArgs M, float px, float py, floa
     int best_feat = 0;
     uint32_t best_t = 0xffffffffu;
     f3 best_q = mk(0.0f, 0.0f, 0.0f);
-    // candidates four at a time: their ids and spheres are contiguous in the
-    // cell's list, so all eight loads are in flight together
-    for (uint32_t i = i0; i < i1; i += 4) {
-        float4 sp[4];
-        uint32_t tt[4];
+    // candidates kMeshRound at a time: their ids, spheres and keys are contiguous in
+    // the cell's list, so the loads are in flight together.  The list is
+    // sorted by `key` (the distance from the cell's centre to the triangle's
+    // bounding sphere, rounded down); key - |x - centre| bounds the distance
+    // from below, so once that passes `reach` no later entry can win.  The
+    // nearest of equal distances is the lowest id (the order of the list
+    // does not matter).
+    bool more = true;
+    for (uint32_t i = i0; i < i1 && more; i += kMeshRound) {
+        float4 sp[kMeshRound];
+        uint32_t tt[kMeshRound];
+        float kk[kMeshRound];
 #pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) {
+        for (uint32_t u = 0; u < kMeshRound; ++u) {
             const uint32_t e = min(i + u, i1 - 1);
             sp[u] = M.cell_spheres[e];
             tt[u] = M.cell_tris[e];
+            kk[u] = M.cell_keys[e];
         }
 #pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) {
-            if (i + u >= i1) break;
+        for (uint32_t u = 0; u < kMeshRound; ++u) {
+            if (i + u >= i1 || !more) break;
+            if (kk[u] - dxc > reach) {
+                more = false;
+                break;
+            }
             const f3 dc = sub(x, mk(sp[u].x, sp[u].y, sp[u].z));
             const float lim = sp[u].w + reach;
             if (dot(dc, dc) > lim * lim * 1.0001f + 1e-12f) continue;
@@ -427,7 +446,7 @@ __device__ __noinline__ float4 mesh_contact(MeshArgs M, float px, float py, floa
             const f3 q = closest_on_tri(x, ld_f3(r), ld_f3(r + 1), ld_f3(r + 2), &feat);
             const f3 dq = sub(x, q);
             const float d2 = dot(dq, dq);
-            if (d2 < best) {
+            if (d2 < best || (d2 == best && best_t != 0xffffffffu && tt[u] < best_t)) {
                 best = d2;
                 reach = sqrtf(best) + slack;
                 best_t = tt[u];
@@ -966,7 +985,7 @@ __device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
                                                     (uint32_t)(by0 + (t / nbx) % nby)) * a.gw.nbx +
                                                    (uint32_t)(bx0 + t % nbx);
                                 const uint32_t at = atomicAdd(a.gw.brick_cnt + a.parity * nbr + b, 1u);
-                                if (at < a.gw.list_cap) a.gw.brick_list[((size_t)a.parity * nbr + b) * a.gw.list_cap + at] = di;
+                                if (at < kBrickList) a.gw.brick_list[((size_t)a.parity * nbr + b) * kBrickList + at] = di;
                             }
                         }
                         int4* dst = a.gw.pool + (size_t)a.parity * a.gw.pool_cap + off;
@@ -1116,8 +1135,9 @@ __device__ __forceinline__ void reduce_brick(const GridWork& gw, uint32_t G, uin
     const int* desc = gw.desc + (size_t)parity * gw.desc_cap * kDescInts;
     const int4* pool = gw.pool + (size_t)parity * gw.pool_cap;
     uint4 acc = make_uint4(0, 0, 0, 0);
-    if (nlisted <= gw.list_cap) {
-        const uint32_t* bl = gw.brick_list + ((size_t)parity * nbr + brick) * gw.list_cap;
+    // a compile-time capacity: with a run-time one this kernel measured 1.8x slower
+    if (nlisted <= kBrickList) {
+        const uint32_t* bl = gw.brick_list + ((size_t)parity * nbr + brick) * kBrickList;
         for (uint32_t base = 0; base < nlisted; base += 64) {
             // lane l fetches box base+l's descriptor: one dependent load chain per 64 boxes
             const uint32_t m = min(64u, nlisted - base);

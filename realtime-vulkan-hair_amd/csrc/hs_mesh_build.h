@@ -25,8 +25,10 @@ struct Built {
     std::vector<float> tris;          // [T][kTriFloats]
     std::vector<float> spheres;       // [T][4]: centroid, radius (rounded outward) of each triangle
     std::vector<uint32_t> cell_start; // [cells + 1]
-    std::vector<uint32_t> cell_tris;  // triangle ids per cell, ascending
+    std::vector<uint32_t> cell_tris;  // triangle ids per cell, nearest bounding sphere first
     std::vector<float> cell_spheres;  // [entries][4]: the bounding sphere of each listed triangle
+    std::vector<float> cell_keys;     // [entries]: |cell centre - sphere centre| - radius, rounded down
+    float half_diag = 0.0f;           // half the cell diagonal, rounded up
     float origin[3] = {0, 0, 0};
     float cell[3] = {1, 1, 1};
     float inv_cell[3] = {1, 1, 1};
@@ -150,11 +152,35 @@ inline bool build(const float* v, uint32_t nv, const uint32_t* t, uint32_t nt, c
                 for (int x = c0[0]; x <= c1[0]; ++x)
                     lists[((uint64_t)z * B.dim[1] + y) * B.dim[0] + x].push_back(f);
     }
+    // each cell's list sorted by a lower bound of the distance from the cell to
+    // the triangle (centre distance minus radius): the kernel stops once the
+    // bound minus the cell's half diagonal passes the best distance so far
+    const double hd = 0.5 * std::sqrt((double)B.cell[0] * B.cell[0] + (double)B.cell[1] * B.cell[1] +
+                                      (double)B.cell[2] * B.cell[2]);
+    B.half_diag = (float)(hd * (1.0 + 1e-5) + 1e-6);
+    std::vector<std::pair<float, uint32_t>> keyed;
     B.cell_start.assign(ncell + 1, 0);
-    for (uint64_t c = 0; c < ncell; ++c) B.cell_start[c + 1] = B.cell_start[c] + (uint32_t)lists[c].size();
     B.cell_tris.clear();
-    B.cell_tris.reserve(B.cell_start[ncell]);
-    for (auto& l : lists) B.cell_tris.insert(B.cell_tris.end(), l.begin(), l.end());
+    B.cell_keys.clear();
+    for (uint64_t cidx = 0; cidx < ncell; ++cidx) {
+        const uint64_t x = cidx % B.dim[0], y = (cidx / B.dim[0]) % B.dim[1], z = cidx / ((uint64_t)B.dim[0] * B.dim[1]);
+        const D3 cc{B.origin[0] + (x + 0.5) * B.cell[0], B.origin[1] + (y + 0.5) * B.cell[1],
+                    B.origin[2] + (z + 0.5) * B.cell[2]};
+        keyed.clear();
+        for (uint32_t f : lists[cidx]) {
+            const float* sp = &B.spheres[(size_t)f * 4];
+            const D3 d{cc.x - sp[0], cc.y - sp[1], cc.z - sp[2]};
+            const double key = std::sqrt(dot(d, d)) - sp[3];
+            // rounded down (and a little more) so the float key never exceeds the true bound
+            keyed.emplace_back((float)(key - 1e-6 * (1.0 + std::fabs(key))), f);
+        }
+        std::sort(keyed.begin(), keyed.end());
+        for (auto& kf : keyed) {
+            B.cell_tris.push_back(kf.second);
+            B.cell_keys.push_back(kf.first);
+        }
+        B.cell_start[cidx + 1] = (uint32_t)B.cell_tris.size();
+    }
     // the spheres again, in list order: a cell's candidates are read as one
     // contiguous stream, without waiting for each id first
     B.cell_spheres.resize(B.cell_tris.size() * 4);
