@@ -293,6 +293,10 @@ int hs_profile_read(hs_ctx* ctx, uint64_t* launches, double* total_ms);
 /* The same total split into the strand step kernel and the voxel reduce
  * kernel (a step with hair-hair friction runs both).  Blocks. */
 int hs_profile_breakdown(hs_ctx* ctx, double* step_kernel_ms, double* reduce_kernel_ms);
+/* Summed time of the mesh pass (the per-point mesh contact kernel that
+ * hs_step runs before the step kernel when a mesh collider is in use; it is
+ * part of hs_profile_read's total, not of the step kernel's).  Blocks. */
+int hs_profile_mesh(hs_ctx* ctx, double* mesh_kernel_ms);
 
 /* ------------------------------------------------------------------------ */
 /* Host-side collider helpers (src/Scene.h:28-38, src/Scene.cpp:110-136)      */

@@ -167,6 +167,10 @@ int drain_profile(hs_ctx* c)
         float ms = 0.0f;
         HS_TRY(c, hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
         c->prof_step_ms += ms;
+        if (ev.mesh) {
+            HS_TRY(c, hipEventElapsedTime(&ms, ev.e[3], ev.e[0]));
+            c->prof_mesh_ms += ms;
+        }
         if (ev.reduce) {
             HS_TRY(c, hipEventElapsedTime(&ms, ev.e[1], ev.e[2]));
             c->prof_reduce_ms += ms;
@@ -194,7 +198,7 @@ int next_events(hs_ctx* c, hs_ctx::Ev** out)
             if (rc != HS_OK) return rc;
         } else {
             hs_ctx::Ev ev{};
-            for (int k = 0; k < 3; ++k) HS_TRY(c, hipEventCreate(&ev.e[k]));
+            for (int k = 0; k < 4; ++k) HS_TRY(c, hipEventCreate(&ev.e[k]));
             c->events.push_back(ev);
         }
     }
@@ -556,7 +560,8 @@ void hs_destroy(hs_ctx* c)
     for (auto& kv : c->meshes) (void)hipFree(kv.second.block);
     c->meshes.clear();
     for (auto& ev : c->events)
-        for (int k = 0; k < 3; ++k) (void)hipEventDestroy(ev.e[k]);
+        for (int k = 0; k < 4; ++k) (void)hipEventDestroy(ev.e[k]);
+    (void)hipFree(c->mesh_f);
     drop_graphs(c);
     for (int k = 0; k < hs_ctx::kPinSlots; ++k) {
         if (c->pin[k]) (void)hipHostFree(c->pin[k]);
@@ -843,13 +848,30 @@ int hs_step(hs_ctx* c, uint32_t id, const hs_time* t, const hs_collider* cols, u
     int rc = build_args(c, *g, t, cols, ncol, p, "hs_step", &a, &gather, &splat);
     if (rc != HS_OK) return rc;
 
+    const bool mesh = a.mesh.tris != nullptr;
+    if (mesh) {
+        // the mesh pass writes every point's contact; the step kernel reads it
+        const size_t need = (size_t)g->N * g->L.Sp * 16;
+        if (need > c->mesh_f_cap) {
+            HS_TRY(c, hipStreamSynchronize(c->stream));
+            (void)hipFree(c->mesh_f);
+            c->mesh_f = nullptr;
+            c->mesh_f_cap = 0;
+            HS_TRY(c, hipMalloc(&c->mesh_f, need));
+            c->mesh_f_cap = need;
+        }
+        a.mesh_f = static_cast<const float4*>(c->mesh_f);
+    }
     hs_ctx::Ev* ev = nullptr;
     if (c->profile) {
         rc = next_events(c, &ev);
         if (rc != HS_OK) return rc;
         ev->reduce = splat;
-        HS_TRY(c, hipEventRecord(ev->e[0], c->stream));
+        ev->mesh = mesh;
+        if (mesh) HS_TRY(c, hipEventRecord(ev->e[3], c->stream));
     }
+    if (mesh) HS_TRY(c, hs::launch_mesh_force(a, c->stream));
+    if (ev) HS_TRY(c, hipEventRecord(ev->e[0], c->stream));
     HS_TRY(c, hs::launch_step(a, gather, splat, c->stream));
     if (ev) HS_TRY(c, hipEventRecord(ev->e[1], c->stream));
     if (splat) {
@@ -960,6 +982,7 @@ int hs_step_batch(hs_ctx* c, const hs_batch_item* items, uint32_t n)
         rc = next_events(c, &ev);
         if (rc != HS_OK) return rc;
         ev->reduce = false;
+        ev->mesh = false;
     }
     if (exec) {
         if (ev) HS_TRY(c, hipEventRecord(ev->e[0], c->stream));
@@ -1225,6 +1248,7 @@ int hs_profile_enable(hs_ctx* c, int on)
     c->prof_launches = 0;
     c->prof_step_ms = 0.0;
     c->prof_reduce_ms = 0.0;
+    c->prof_mesh_ms = 0.0;
     return HS_OK;
 }
 
@@ -1234,7 +1258,7 @@ int hs_profile_read(hs_ctx* c, uint64_t* launches, double* total_ms)
     int rc = drain_profile(c);
     if (rc != HS_OK) return rc;
     *launches = c->prof_launches;
-    *total_ms = c->prof_step_ms + c->prof_reduce_ms;
+    *total_ms = c->prof_step_ms + c->prof_reduce_ms + c->prof_mesh_ms;
     return HS_OK;
 }
 
@@ -1245,6 +1269,15 @@ int hs_profile_breakdown(hs_ctx* c, double* step_kernel_ms, double* reduce_kerne
     if (rc != HS_OK) return rc;
     *step_kernel_ms = c->prof_step_ms;
     *reduce_kernel_ms = c->prof_reduce_ms;
+    return HS_OK;
+}
+
+int hs_profile_mesh(hs_ctx* c, double* mesh_kernel_ms)
+{
+    if (!c || !mesh_kernel_ms) return HS_ERR_INVALID;
+    int rc = drain_profile(c);
+    if (rc != HS_OK) return rc;
+    *mesh_kernel_ms = c->prof_mesh_ms;
     return HS_OK;
 }
 

@@ -100,16 +100,20 @@ struct hs_ctx {
     uint64_t bytes = 0;
     std::map<uint32_t, hsi::Mesh> meshes;
     uint32_t next_mesh = 1;
-    // kernel timing: pooled events (step start, step end, reduce end) per step
+    // kernel timing: pooled events (step start, step end, reduce end, mesh
+    // pass start) per step
     bool profile = false;
     struct Ev {
-        hipEvent_t e[3];
-        bool reduce;
+        hipEvent_t e[4];
+        bool reduce, mesh;
     };
     std::vector<Ev> events;
     size_t events_used = 0;
     uint64_t prof_launches = 0;
-    double prof_step_ms = 0.0, prof_reduce_ms = 0.0;
+    double prof_step_ms = 0.0, prof_reduce_ms = 0.0, prof_mesh_ms = 0.0;
+    // hs_step with a mesh: the mesh pass's per-point result (grow-only)
+    void* mesh_f = nullptr;
+    size_t mesh_f_cap = 0;
     // hs_step_batch: the step's arguments (StepArgs per groom, ellipsoid
     // colliders, workgroup/brick prefix arrays) are laid out in one blob,
     // written into a pinned host slot and copied to `dev_blob` on the stream.

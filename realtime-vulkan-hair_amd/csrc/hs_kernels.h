@@ -76,6 +76,7 @@ struct MeshArgs {
 // Collider mode of a step kernel: which colliders beyond the sphere it handles.
 constexpr int kColEll = 1;
 constexpr int kColMesh = 2;
+constexpr int kColMeshPre = 4;  // mesh contact read from a k_mesh_force result (StepArgs::mesh_f)
 
 struct StepArgs {
     float* pos;           // [3][N][Sp]
@@ -99,6 +100,7 @@ struct StepArgs {
     uint32_t ncol;        // colliders in use (0: none, 1: the sphere, >1: + ellipsoids)
     const Collider* cols; // device copy of all ncol colliders (read for ellipsoids 1..ncol-1)
     MeshArgs mesh;        // triangle-mesh collider (mesh.tris null: none)
+    const float4* mesh_f; // [N][Sp] mesh contact of the stored positions (launch_mesh_force), or null
     // tile choices (results do not depend on them)
     uint32_t win_gather;  // voxel cells of the gather window (<= kWinCells)
     uint32_t win_splat;   // voxel cells of a splat window (<= kWinCells / HS_VOXEL_COPIES)
@@ -113,6 +115,8 @@ struct UnpackGrid {
 
 hipError_t launch_step(const StepArgs& a, bool gather, bool splat, hipStream_t st);
 hipError_t launch_reduce(const GridWork& gw, uint32_t G, uint32_t parity, hipStream_t st);
+// Mesh contact of every stored position of `a` into a.mesh_f (N * Sp float4).
+hipError_t launch_mesh_force(const StepArgs& a, hipStream_t st);
 // Many grooms per launch: per-groom arguments in device memory; wg_start /
 // brick_start are prefix sums (ngroom + 1 entries) of workgroups / bricks.
 hipError_t launch_step_batch(const StepArgs* args_dev, const uint32_t* wg_start_dev, uint32_t ngroom,

@@ -152,6 +152,56 @@ inline bool build(const float* v, uint32_t nv, const uint32_t* t, uint32_t nt, c
                 for (int x = c0[0]; x <= c1[0]; ++x)
                     lists[((uint64_t)z * B.dim[1] + y) * B.dim[0] + x].push_back(f);
     }
+    // Cells entirely outside the mesh need no list: the contact rule requires
+    // the point to be inside.  A cell is "surface" when a triangle's bounds,
+    // grown by a small epsilon, reach it; cells joined to the grid's border by
+    // non-surface cells hold no surface point and lie outside (crossing into
+    // the mesh means crossing its surface).  The epsilon keeps every point of
+    // such a cell well away from the surface, where the float inside test is
+    // unambiguous.
+    {
+        std::vector<uint8_t> surf(ncell, 0), outside(ncell, 0);
+        for (uint32_t f = 0; f < nt; ++f) {
+            const float* r = &B.tris[(size_t)f * kTriFloats];
+            int c0[3], c1[3];
+            for (int k = 0; k < 3; ++k) {
+                const double lo = std::min({(double)r[k], (double)r[4 + k], (double)r[8 + k]});
+                const double hi = std::max({(double)r[k], (double)r[4 + k], (double)r[8 + k]});
+                const double eps = 1e-4 * (1.0 + std::max(std::fabs(lo), std::fabs(hi))) + 0.01 * B.cell[k];
+                c0[k] = std::max(0, (int)std::floor((lo - eps - B.origin[k]) / B.cell[k]));
+                c1[k] = std::min((int)B.dim[k] - 1, (int)std::floor((hi + eps - B.origin[k]) / B.cell[k]));
+            }
+            for (int z = c0[2]; z <= c1[2]; ++z)
+                for (int y = c0[1]; y <= c1[1]; ++y)
+                    for (int x = c0[0]; x <= c1[0]; ++x) surf[((uint64_t)z * B.dim[1] + y) * B.dim[0] + x] = 1;
+        }
+        std::vector<uint64_t> stack;
+        auto seed = [&](uint64_t c) {
+            if (!surf[c] && !outside[c]) {
+                outside[c] = 1;
+                stack.push_back(c);
+            }
+        };
+        const uint32_t X = B.dim[0], Y = B.dim[1], Z = B.dim[2];
+        for (uint32_t z = 0; z < Z; ++z)
+            for (uint32_t y = 0; y < Y; ++y)
+                for (uint32_t x = 0; x < X; ++x)
+                    if (x == 0 || y == 0 || z == 0 || x == X - 1 || y == Y - 1 || z == Z - 1)
+                        seed(((uint64_t)z * Y + y) * X + x);
+        while (!stack.empty()) {
+            const uint64_t c = stack.back();
+            stack.pop_back();
+            const uint32_t x = (uint32_t)(c % X), y = (uint32_t)((c / X) % Y), z = (uint32_t)(c / ((uint64_t)X * Y));
+            if (x > 0) seed(c - 1);
+            if (x + 1 < X) seed(c + 1);
+            if (y > 0) seed(c - X);
+            if (y + 1 < Y) seed(c + X);
+            if (z > 0) seed(c - (uint64_t)X * Y);
+            if (z + 1 < Z) seed(c + (uint64_t)X * Y);
+        }
+        for (uint64_t c = 0; c < ncell; ++c)
+            if (outside[c]) lists[c].clear();
+    }
     // each cell's list sorted by a lower bound of the distance from the cell to
     // the triangle (centre distance minus radius): the kernel stops once the
     // bound minus the cell's half diagonal passes the best distance so far

@@ -183,6 +183,7 @@ _SIGS = {
     "hs_profile_enable": (C.c_int, [_P, C.c_int]),
     "hs_profile_read": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
     "hs_profile_breakdown": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "hs_profile_mesh": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "hs_collider_from_trs": (None, [C.POINTER(HsCollider), C.POINTER(C.c_float), C.POINTER(C.c_float),
                                     C.POINTER(C.c_float)]),
     "hs_collider_translate": (None, [C.POINTER(HsCollider), C.POINTER(C.c_float)]),
@@ -379,6 +380,12 @@ class Context:
         b = C.c_double()
         self._check(lib().hs_profile_breakdown(self._h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def profile_mesh(self):
+        """Summed ms of the mesh pass (hs_step with a mesh collider)."""
+        a = C.c_double()
+        self._check(lib().hs_profile_mesh(self._h, C.byref(a)))
+        return a.value
 
     def groom(self, strands_aos, num_points, *, strand_length=2.5, seg_length=None, wind_noise=None,
               noise_dir=(1.0, 0.0, 0.0), grid_dim=0, grid_origin=(0.0, 0.0, 0.0), grid_h=1.0,
