@@ -429,10 +429,41 @@ struct BatchLaunch {
 
 int collider_mode(uint8_t v) { return ((v & 4) ? hs::kColEll : 0) | ((v & 8) ? hs::kColMesh : 0); }
 
-hipError_t enqueue_batch(const hs_ctx* c, const std::vector<BatchLaunch>& launches, size_t bytes, int slot,
-                         hipStream_t st)
+// The batch's mesh pass: which StepArgs (indices into the blob's array) have
+// a mesh, and the prefix sums of their blocks, both in the blob.
+struct MeshPass {
+    uint64_t idx_off = 0, blk_off = 0;
+    uint32_t nm = 0, blocks = 0;
+};
+
+hipError_t launch_mesh_pass(const hs_ctx* c, const MeshPass& M, hipStream_t st)
+{
+    if (M.nm == 0) return hipSuccess;
+    return hs::launch_mesh_force_batch(reinterpret_cast<const hs::StepArgs*>(c->dev_blob),
+                                       reinterpret_cast<const uint32_t*>(c->dev_blob + M.idx_off),
+                                       reinterpret_cast<const uint32_t*>(c->dev_blob + M.blk_off), M.nm, M.blocks, st);
+}
+
+// Room for `bytes` of mesh pass results (grow-only; the old buffer is freed
+// once the stream is done with it).
+int ensure_mesh_f(hs_ctx* c, size_t bytes)
+{
+    if (bytes <= c->mesh_f_cap) return HS_OK;
+    HS_TRY(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(c->mesh_f);
+    c->mesh_f = nullptr;
+    c->mesh_f_cap = 0;
+    HS_TRY(c, hipMalloc(&c->mesh_f, bytes));
+    c->mesh_f_cap = bytes;
+    return HS_OK;
+}
+
+hipError_t enqueue_batch(const hs_ctx* c, const std::vector<BatchLaunch>& launches, const MeshPass& mp, size_t bytes,
+                         int slot, hipStream_t st)
 {
     hipError_t e = hipMemcpyAsync(c->dev_blob, c->pin[slot], bytes, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return e;
+    e = launch_mesh_pass(c, mp, st);
     if (e != hipSuccess) return e;
     for (const BatchLaunch& L : launches) {
         e = hs::launch_step_batch(reinterpret_cast<const hs::StepArgs*>(c->dev_blob + L.args_off),
@@ -454,7 +485,7 @@ hipError_t enqueue_batch(const hs_ctx* c, const std::vector<BatchLaunch>& launch
 constexpr size_t kMaxGraphs = 16;
 
 int batch_graph(hs_ctx* c, const std::vector<uint64_t>& key, const std::vector<BatchLaunch>& launches,
-                size_t bytes, int slot, hipGraphExec_t* out)
+                const MeshPass& mp, size_t bytes, int slot, hipGraphExec_t* out)
 {
     hs_ctx::BatchGraph* hit = nullptr;
     for (auto& g : c->graph_cache)
@@ -476,7 +507,7 @@ int batch_graph(hs_ctx* c, const std::vector<uint64_t>& key, const std::vector<B
     if (!hit->exec[slot]) {
         if (!c->cap_stream) HS_TRY(c, hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
         HS_TRY(c, hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeRelaxed));
-        const hipError_t le = enqueue_batch(c, launches, bytes, slot, c->cap_stream);
+        const hipError_t le = enqueue_batch(c, launches, mp, bytes, slot, c->cap_stream);
         hipGraph_t graph = nullptr;
         const hipError_t ce = hipStreamEndCapture(c->cap_stream, &graph);
         if (le != hipSuccess) {
@@ -851,15 +882,8 @@ int hs_step(hs_ctx* c, uint32_t id, const hs_time* t, const hs_collider* cols, u
     const bool mesh = a.mesh.tris != nullptr;
     if (mesh) {
         // the mesh pass writes every point's contact; the step kernel reads it
-        const size_t need = (size_t)g->N * g->L.Sp * 16;
-        if (need > c->mesh_f_cap) {
-            HS_TRY(c, hipStreamSynchronize(c->stream));
-            (void)hipFree(c->mesh_f);
-            c->mesh_f = nullptr;
-            c->mesh_f_cap = 0;
-            HS_TRY(c, hipMalloc(&c->mesh_f, need));
-            c->mesh_f_cap = need;
-        }
+        rc = ensure_mesh_f(c, (size_t)g->N * g->L.Sp * sizeof(float4));
+        if (rc != HS_OK) return rc;
         a.mesh_f = static_cast<const float4*>(c->mesh_f);
     }
     hs_ctx::Ev* ev = nullptr;
@@ -946,6 +970,31 @@ int hs_step_batch(hs_ctx* c, const hs_batch_item* items, uint32_t n)
         idx.insert(idx.end(), br.begin(), br.end());
         launches.push_back(L);
     }
+    // mesh pass: the grooms with a mesh, their blocks, and their slices of
+    // the shared result buffer
+    MeshPass mp;
+    size_t mesh_f_bytes = 0;
+    std::vector<size_t> mesh_at(n, 0);
+    {
+        std::vector<uint32_t> midx, mblk{0};
+        for (size_t j = 0; j < order.size(); ++j) {
+            const hs::StepArgs& a = args[order[j]];
+            if (!a.mesh.tris) continue;
+            const uint64_t plane = (uint64_t)a.N * a.Sp;
+            midx.push_back((uint32_t)j);
+            mblk.push_back(mblk.back() + (uint32_t)((plane + 255) / 256));
+            mesh_at[order[j]] = mesh_f_bytes;
+            mesh_f_bytes += plane * sizeof(float4);
+        }
+        if (!midx.empty()) {
+            mp.nm = (uint32_t)midx.size();
+            mp.blocks = mblk.back();
+            mp.idx_off = idx.size();
+            idx.insert(idx.end(), midx.begin(), midx.end());
+            mp.blk_off = idx.size();
+            idx.insert(idx.end(), mblk.begin(), mblk.end());
+        }
+    }
     const uint64_t args_bytes = align256(order.size() * sizeof(hs::StepArgs));
     const uint64_t ell_bytes = align256(ell.size() * sizeof(hs::Collider));
     const uint64_t bytes = args_bytes + ell_bytes + idx.size() * sizeof(uint32_t);
@@ -953,10 +1002,20 @@ int hs_step_batch(hs_ctx* c, const hs_batch_item* items, uint32_t n)
         L.wg_off = args_bytes + ell_bytes + L.wg_off * sizeof(uint32_t);
         L.br_off = args_bytes + ell_bytes + L.br_off * sizeof(uint32_t);
     }
+    mp.idx_off = args_bytes + ell_bytes + mp.idx_off * sizeof(uint32_t);
+    mp.blk_off = args_bytes + ell_bytes + mp.blk_off * sizeof(uint32_t);
     key.push_back(bytes);
     key.push_back(c->tile.xcd_order);
+    key.push_back(mp.nm);
     int rc = ensure_blob(c, bytes);
     if (rc != HS_OK) return rc;
+    if (mp.nm) {
+        rc = ensure_mesh_f(c, mesh_f_bytes);
+        if (rc != HS_OK) return rc;
+        for (uint32_t k = 0; k < n; ++k)
+            if (args[k].mesh.tris)
+                args[k].mesh_f = reinterpret_cast<const float4*>(static_cast<uint8_t*>(c->mesh_f) + mesh_at[k]);
+    }
     const int slot = c->slot_next;
     c->slot_next = (slot + 1) % hs_ctx::kPinSlots;
     if (c->slot_used[slot]) HS_TRY(c, hipEventSynchronize(c->slot_done[slot]));
@@ -972,7 +1031,7 @@ int hs_step_batch(hs_ctx* c, const hs_batch_item* items, uint32_t n)
 
     hipGraphExec_t exec = nullptr;
     if (c->graphs) {
-        rc = batch_graph(c, key, launches, bytes, slot, &exec);
+        rc = batch_graph(c, key, launches, mp, bytes, slot, &exec);
         if (rc != HS_OK) return rc;
     }
     hs_ctx::Ev* ev = nullptr;
@@ -985,12 +1044,18 @@ int hs_step_batch(hs_ctx* c, const hs_batch_item* items, uint32_t n)
         ev->mesh = false;
     }
     if (exec) {
+        // one graph: the mesh pass's time is counted with the step kernels
         if (ev) HS_TRY(c, hipEventRecord(ev->e[0], c->stream));
         HS_TRY(c, hipGraphLaunch(exec, c->stream));
         if (ev) HS_TRY(c, hipEventRecord(ev->e[1], c->stream));
     } else if (ev) {
         // eager launches: split the timing into step kernels and reduce kernels
         HS_TRY(c, hipMemcpyAsync(c->dev_blob, c->pin[slot], bytes, hipMemcpyHostToDevice, c->stream));
+        if (mp.nm) {
+            ev->mesh = true;
+            HS_TRY(c, hipEventRecord(ev->e[3], c->stream));
+            HS_TRY(c, launch_mesh_pass(c, mp, c->stream));
+        }
         HS_TRY(c, hipEventRecord(ev->e[0], c->stream));
         for (const BatchLaunch& L : launches)
             HS_TRY(c, hs::launch_step_batch(reinterpret_cast<const hs::StepArgs*>(c->dev_blob + L.args_off),
@@ -1008,7 +1073,7 @@ int hs_step_batch(hs_ctx* c, const hs_batch_item* items, uint32_t n)
             HS_TRY(c, hipEventRecord(ev->e[2], c->stream));
         }
     } else {
-        HS_TRY(c, enqueue_batch(c, launches, bytes, slot, c->stream));
+        HS_TRY(c, enqueue_batch(c, launches, mp, bytes, slot, c->stream));
     }
     HS_TRY(c, hipEventRecord(c->slot_done[slot], c->stream));
     c->slot_used[slot] = true;

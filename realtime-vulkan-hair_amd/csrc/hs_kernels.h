@@ -75,8 +75,7 @@ struct MeshArgs {
 
 // Collider mode of a step kernel: which colliders beyond the sphere it handles.
 constexpr int kColEll = 1;
-constexpr int kColMesh = 2;
-constexpr int kColMeshPre = 4;  // mesh contact read from a k_mesh_force result (StepArgs::mesh_f)
+constexpr int kColMesh = 2;  // mesh contact read from the mesh pass's result (StepArgs::mesh_f)
 
 struct StepArgs {
     float* pos;           // [3][N][Sp]
@@ -115,8 +114,13 @@ struct UnpackGrid {
 
 hipError_t launch_step(const StepArgs& a, bool gather, bool splat, hipStream_t st);
 hipError_t launch_reduce(const GridWork& gw, uint32_t G, uint32_t parity, hipStream_t st);
-// Mesh contact of every stored position of `a` into a.mesh_f (N * Sp float4).
+// Mesh pass: mesh contact of every stored position of `a` into a.mesh_f
+// (N * Sp float4).  Runs before a step kernel that reads it.
 hipError_t launch_mesh_force(const StepArgs& a, hipStream_t st);
+// The same for many grooms: args_dev[idx_dev[m]] for m < nm; blk_start_dev
+// holds prefix sums of ceil(N * Sp / 256) blocks (nm + 1 entries).
+hipError_t launch_mesh_force_batch(const StepArgs* args_dev, const uint32_t* idx_dev, const uint32_t* blk_start_dev,
+                                   uint32_t nm, uint32_t total_blocks, hipStream_t st);
 // Many grooms per launch: per-groom arguments in device memory; wg_start /
 // brick_start are prefix sums (ngroom + 1 entries) of workgroups / bricks.
 hipError_t launch_step_batch(const StepArgs* args_dev, const uint32_t* wg_start_dev, uint32_t ngroom,

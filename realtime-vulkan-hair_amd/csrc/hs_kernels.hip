@@ -382,11 +382,15 @@ __device__ __forceinline__ f3 ld_f3(const float4* p)
 // ties); contact when |x - q|^2 < margin^2 and dot(x - q, n) < 0 for the
 // pseudo-normal n of q's feature (Baerentzen & Aanaes, IEEE TVCG 2005).
 // Returns (k |x - q|) normalize(q - x) in xyz and 1 in w on contact, else 0.
-// No reference counterpart.  The step kernel calls the out-of-line copy; the
-// mesh pass (k_mesh_force) inlines it.
-constexpr uint32_t kMeshRound = 8;  // mesh candidates loaded per round
+// No reference counterpart.  Evaluated by the mesh pass (k_mesh_force /
+// k_mesh_force_batch) for every stored point before the step kernel, which
+// reads the result.
+#ifndef HS_MESH_PASS_ROUND
+#define HS_MESH_PASS_ROUND 2
+#endif
+constexpr uint32_t kMeshRound = HS_MESH_PASS_ROUND;  // mesh candidates loaded per round
 
-__device__ __forceinline__ float4 mesh_contact_body(const MeshArgs& M, float px, float py, float pz, float k)
+__device__ __forceinline__ float4 mesh_contact(const MeshArgs& M, float px, float py, float pz, float k)
 {
     const f3 x = mk(px, py, pz);
     const float4 none = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -466,24 +470,24 @@ __device__ __forceinline__ float4 mesh_contact_body(const MeshArgs& M, float px,
     return make_float4(c.x, c.y, c.z, 1.0f);
 }
 
-__device__ __noinline__ float4 mesh_contact(MeshArgs M, float px, float py, float pz, float k)
+// Mesh pass: the mesh contact of every stored point (the positions a step
+// starts from), one thread per point, into out[N][Sp].  Apart from the step
+// kernel, whose register budget holds it to two waves per SIMD, this runs at
+// full occupancy, which hides the scattered triangle loads.
+__device__ __forceinline__ void mesh_point(const MeshArgs& M, const float* __restrict__ pos, uint32_t plane, float k,
+                                           float4* __restrict__ out, uint32_t i)
 {
-    return mesh_contact_body(M, px, py, pz, k);
+    out[i] = mesh_contact(M, pos[i], pos[plane + i], pos[2 * (size_t)plane + i], k);
 }
 
-// Mesh pass: the mesh contact of every stored point (the positions a step
-// starts from), one thread per point, into out[N][Sp].  The step kernel's
-// register budget holds it to two waves per SIMD; this kernel runs at full
-// occupancy, which hides the scattered triangle loads.
 __global__ __launch_bounds__(256) void k_mesh_force(MeshArgs M, const float* __restrict__ pos, uint32_t plane,
                                                     float k, float4* __restrict__ out)
 {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= plane) return;
-    out[i] = mesh_contact_body(M, pos[i], pos[plane + i], pos[2 * (size_t)plane + i], k);
+    if (i < plane) mesh_point(M, pos, plane, k, out, i);
 }
 
-// `mf`: the mesh pass's result for this point (read when COL has kColMeshPre)
+// `mf`: the mesh pass's result for this point (used when COL has kColMesh)
 template <int COL>
 __device__ __forceinline__ f3 force_at(f3 x, f3 fs, const StepArgs& a, float4 mf)
 {
@@ -520,14 +524,7 @@ __device__ __forceinline__ f3 force_at(f3 x, f3 fs, const StepArgs& a, float4 mf
     }
     if (COL & kColMesh) {
         // the mesh counts as one more collider, after the reference's ones
-        const float4 c = mesh_contact(a.mesh, x.x, x.y, x.z, a.k);
-        if (c.w != 0.0f) {
-            acc = add(acc, mk(c.x, c.y, c.z));
-            hits += 1;
-        }
-    }
-    if (COL & kColMeshPre) {
-        // the same contribution, computed by k_mesh_force
+        // (its contact computed by the mesh pass)
         if (mf.w != 0.0f) {
             acc = add(acc, mk(mf.x, mf.y, mf.z));
             hits += 1;
@@ -809,7 +806,7 @@ __device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
                 }
                 // predict (compute.comp:187)
                 float4 mf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                if (COL & kColMeshPre) mf = a.mesh_f[(size_t)(f + t) * Sp + s];
+                if (COL & kColMesh) mf = a.mesh_f[(size_t)(f + t) * Sp + s];
                 const f3 fc = force_at<COL>(x, fs, a, mf);
                 U[t] = add(add(x, smul(a.dt, u)), smul(a.dt2, fc));
             }
@@ -1098,6 +1095,21 @@ __global__ __launch_bounds__(kW) void k_step_batch(const StepArgs* __restrict__ 
     step_body<GATHER, SPLAT, COL>(a, gwg - wg_start[g]);
 }
 
+// Mesh pass over many grooms: args[idx[m]] is the m-th groom with a mesh,
+// blk_start[m] its first block (prefix sums, nm + 1 entries).  Arguments are
+// read from device memory, so a captured launch follows later calls' values.
+__global__ __launch_bounds__(256) void k_mesh_force_batch(const StepArgs* __restrict__ args,
+                                                          const uint32_t* __restrict__ idx,
+                                                          const uint32_t* __restrict__ blk_start, uint32_t nm)
+{
+    if (blockIdx.x >= blk_start[nm]) return;
+    const uint32_t m = find_groom(blk_start, nm, blockIdx.x);
+    const StepArgs& a = args[idx[m]];
+    const uint32_t plane = a.N * a.Sp;
+    const uint32_t i = (blockIdx.x - blk_start[m]) * 256u + threadIdx.x;
+    if (i < plane) mesh_point(a.mesh, a.pos, plane, a.k, const_cast<float4*>(a.mesh_f), i);
+}
+
 // ---------------------------------------------------------------------------
 // Reduce: partial boxes -> grid (GridCell ints + float cells); clears the grid.
 // ---------------------------------------------------------------------------
@@ -1319,10 +1331,6 @@ static void launch_step_e(const StepArgs& a, int col, uint32_t blocks, hipStream
     case 0: hipLaunchKernelGGL((k_step<G, S, 0>), dim3(blocks), dim3(kW), 0, st, a); break;
     case kColEll: hipLaunchKernelGGL((k_step<G, S, kColEll>), dim3(blocks), dim3(kW), 0, st, a); break;
     case kColMesh: hipLaunchKernelGGL((k_step<G, S, kColMesh>), dim3(blocks), dim3(kW), 0, st, a); break;
-    case kColMeshPre: hipLaunchKernelGGL((k_step<G, S, kColMeshPre>), dim3(blocks), dim3(kW), 0, st, a); break;
-    case kColEll | kColMeshPre:
-        hipLaunchKernelGGL((k_step<G, S, kColEll | kColMeshPre>), dim3(blocks), dim3(kW), 0, st, a);
-        break;
     default:
         hipLaunchKernelGGL((k_step<G, S, kColEll | kColMesh>), dim3(blocks), dim3(kW), 0, st, a);
         break;
@@ -1332,7 +1340,8 @@ static void launch_step_e(const StepArgs& a, int col, uint32_t blocks, hipStream
 hipError_t launch_step(const StepArgs& a, bool gather, bool splat, hipStream_t st)
 {
     const uint32_t blocks = ((a.nwg + 7) / 8) * 8;
-    const int col = (a.ncol > 1 ? kColEll : 0) | (a.mesh.tris ? (a.mesh_f ? kColMeshPre : kColMesh) : 0);
+    if (a.mesh.tris && !a.mesh_f) return hipErrorInvalidValue;
+    const int col = (a.ncol > 1 ? kColEll : 0) | (a.mesh.tris ? kColMesh : 0);
     if (gather) {
         if (splat)
             launch_step_e<true, true>(a, col, blocks, st);
@@ -1397,8 +1406,16 @@ hipError_t launch_reduce_batch(const StepArgs* args_dev, const uint32_t* brick_s
 hipError_t launch_mesh_force(const StepArgs& a, hipStream_t st)
 {
     const uint32_t plane = a.N * a.Sp;
+    if (!a.mesh.tris || !a.mesh_f) return hipErrorInvalidValue;
     float4* out = const_cast<float4*>(a.mesh_f);
     hipLaunchKernelGGL(k_mesh_force, dim3((plane + 255) / 256), dim3(256), 0, st, a.mesh, a.pos, plane, a.k, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_mesh_force_batch(const StepArgs* args_dev, const uint32_t* idx_dev, const uint32_t* blk_start_dev,
+                                   uint32_t nm, uint32_t total_blocks, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mesh_force_batch, dim3(total_blocks), dim3(256), 0, st, args_dev, idx_dev, blk_start_dev, nm);
     return hipGetLastError();
 }
 

@@ -151,7 +151,7 @@ def head_mesh(subdiv=5, scale=(1.0, 1.2, 1.0)):
     return icosphere(subdiv, scale)
 
 
-def mesh_head_scene(name, seed, S, N, G, subdiv, scale=(1.0, 1.2, 1.0), margin=0.05, mesh_grid=(64, 76, 64),
+def mesh_head_scene(name, seed, S, N, G, subdiv, scale=(1.0, 1.2, 1.0), margin=0.05, mesh_grid=(96, 114, 96),
                     steps=120):
     """Roots on the scaled upper hemisphere (the head's scalp), strands along
     the surface normal; wind, noise, orbiting sphere and grid as plain-100k,

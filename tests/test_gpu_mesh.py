@@ -103,3 +103,51 @@ def test_mesh_errors(ctx):
         gpu.step(wl.dt, 0.0, [], hairsim.step_params(mesh_id=999))
     assert e.value.code == hairsim.HS_ERR_NOT_FOUND
     gpu.destroy()
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_two_mesh_grooms_batched_bitwise(ctx, graphs):
+    """Two grooms of different shapes share one mesh in one batch; the mesh
+    pass covers both.  Under graph replay the spring constant changes every
+    step: the captured mesh pass must take each call's value."""
+    wls = [_scene(S=600, N=12, seed=21), _scene(S=500, N=16, seed=22)]
+    V, T = wls[0].mesh
+    mid = ctx.mesh(V, T, grid_dim=wls[0].mesh_grid, margin=wls[0].mesh_margin)
+    recs = oracle.mesh_prepare(V, T)
+    gs = [w.grooms[0] for w in wls]
+    gpus = [ctx.groom(g.strands, g.N, seg_length=g.seg, wind_noise=g.noise, grid_dim=g.grid_dim,
+                      grid_origin=g.grid_origin, grid_h=g.grid_h) for g in gs]
+    refs = [g.strands.copy() for g in gs]
+    grids = [None, None]
+    ctx.set_batch_graphs(graphs)
+    try:
+        for k in range(12):
+            items, ops = [], []
+            for j, (w, g) in enumerate(zip(wls, gs)):
+                cols = [scenes.sphere_collider(w.sphere(k))]
+                kk = 1900.0 + 150.0 * ((k + j) % 4)
+                items.append((gpus[j], w.dt, float(w.time(k)), cols,
+                              hairsim.step_params(wind=w.wind(k), damping=w.damping, spring_k=kk,
+                                                  sphere_radius=w.sphere_radius, friction=w.friction,
+                                                  num_colliders=1, hair_hair=True, mesh_id=mid)))
+                ops.append(oracle.Params(dt=w.dt, wind=w.wind(k), damping=w.damping, spring_k=kk,
+                                         sphere_radius=w.sphere_radius, friction=w.friction, hair_hair=True,
+                                         grid_dim=g.grid_dim, grid_origin=g.grid_origin, grid_h=g.grid_h,
+                                         colliders=np.stack(cols), seg_length=g.seg, wind_noise=g.noise,
+                                         noise_dir=g.noise_dir, mesh=recs, mesh_margin=w.mesh_margin))
+            ctx.step_batch(items)
+            for j, g in enumerate(gs):
+                grids[j] = oracle.step32_mt(ops[j], refs[j], g.N, grids[j], 8)
+        for j, g in enumerate(gs):
+            out, _ = gpus[j].read()
+            o = out.reshape(-1, 3, g.N, 4).copy()
+            r = refs[j].reshape(-1, 3, g.N, 4).copy()
+            o[:, 2] = 0
+            r[:, 2] = 0
+            assert_bitwise(o, r, f"strands of groom {j}", g.N)
+            assert np.array_equal(gpus[j].read_grid(), grids[j].reshape(-1))
+    finally:
+        ctx.set_batch_graphs(False)
+        for gp in gpus:
+            gp.destroy()
+        ctx.mesh_destroy(mid)
