@@ -137,6 +137,7 @@ def bench_single(args, hairsim, scenes, wl):
     ctx.sync()
     n_launch, _ = ctx.profile_read()
     step_ms, reduce_ms = ctx.profile_breakdown()
+    mesh_ms = ctx.profile_mesh()
     ctx.profile_enable(False)
     out, _ = groom.read()
     near = groom.grid_near_limit() if g.grid_dim and wl.hair_hair else 0
@@ -151,10 +152,17 @@ def bench_single(args, hairsim, scenes, wl):
     n = max(1, n_launch)
     step_us = step_ms / n * 1e3
     reduce_us = reduce_ms / n * 1e3
+    mesh_us = mesh_ms / n * 1e3
     b_step_kernel = scenes.strand_bytes(g.S, g.N)
     b_grid = scenes.algorithmic_bytes(g.S, g.N, g.grid_dim, wl.hair_hair) - b_step_kernel
+    b_mesh = 0
+    if wl.mesh is not None:
+        # mesh pass: reads every stored position (12 B), writes its contact
+        # (16 B); the step kernel reads the contact of points 1..N-1
+        b_mesh = 28 * g.S * g.N
+        b_step_kernel += 16 * g.S * (g.N - 1)
     achieved = b_step_kernel / (step_us * 1e-6) / 1e9
-    whole = (b_step_kernel + b_grid) / ((step_us + reduce_us) * 1e-6) / 1e9
+    whole = (b_step_kernel + b_grid + b_mesh) / ((step_us + reduce_us + mesh_us) * 1e-6) / 1e9
     res = {
         "metric": "strand-point updates/sec",
         "value": value,
@@ -179,6 +187,23 @@ def bench_single(args, hairsim, scenes, wl):
                      "whole_step_achieved": whole, "whole_step_frac": whole / PEAK_HBM_GBS},
         "grid_cells_near_int32_limit": near,
     }
+    if b_mesh:
+        mesh_gbs = b_mesh / (mesh_us * 1e-6) / 1e9
+        mesh = {"kernel": "k_mesh_force", "kernel_avg_us": mesh_us, "algorithmic_bytes_per_launch": b_mesh,
+                "achieved": mesh_gbs, "frac": mesh_gbs / PEAK_HBM_GBS}
+        if mesh_us > step_us:
+            # the mesh pass is the dominant kernel: it heads the roofline
+            rl = res["roofline"]
+            res["roofline"] = {"bound": "hbm", "achieved": mesh_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                               "frac": mesh_gbs / PEAK_HBM_GBS, "traffic": load_traffic(wl.name + "-mesh"),
+                               "kernel": "k_mesh_force", "kernel_avg_us": mesh_us,
+                               "algorithmic_bytes_per_launch": b_mesh,
+                               "step_kernel": {k: rl[k] for k in ("achieved", "frac", "kernel_avg_us",
+                                                                  "algorithmic_bytes_per_launch", "traffic")},
+                               "reduce_kernel_avg_us": reduce_us, "reduce_algorithmic_bytes": b_grid,
+                               "whole_step_achieved": whole, "whole_step_frac": whole / PEAK_HBM_GBS}
+        else:
+            res["roofline"]["mesh_pass"] = mesh
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(wl)
     print(json.dumps(res), flush=True)
