@@ -133,11 +133,16 @@ typedef struct hs_step_params {
     uint32_t num_colliders;  /* colliders used (NUM_COLLIDERS 6)               */
     uint32_t hair_hair;      /* 1: splat + grid friction; 0: skip the grid     */
     uint32_t mesh_id;        /* 0: none; else a mesh collider (hs_mesh_create) */
+    uint32_t forcing;        /* wind forcing the shader carries commented out:
+                              * 0 none (the reference), 1 compute.comp:150,
+                              * 2 compute.comp:151 (fbm-modulated); sin/cos
+                              * are one fixed binary32 algorithm, absolute
+                              * error < 1e-7 for |x| <= 1e5                  */
 } hs_step_params;
 
 /* Fill `p` with the reference's constants (DAMPING 0.998, k 1900, sphere
  * radius 1.0, vmax 10, friction 0.08, gravity (0,-9.8,0), 6 colliders,
- * hair-hair on, no wind, no mesh). */
+ * hair-hair on, no wind, no mesh, no forcing). */
 void hs_step_params_reference(hs_step_params* p);
 
 /* Fill `d` with the reference groom defaults for S strands of N points:

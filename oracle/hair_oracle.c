@@ -18,9 +18,44 @@
 #error "the oracle must not be built with -ffast-math"
 #endif
 
+/* sin and cos for the forcing (compute.comp:150-151 use GLSL sin/cos, whose
+ * precision the language leaves open): k = nearest multiple of pi/2 as
+ * floor(x*(2/pi) + 0.5); r = ((x - k*P1) - k*P2) - k*P3 with P1 = 201/128
+ * and P2 of 8 significant bits (k*P1, k*P2 exact for |k| < 2^16) and P3 the
+ * float remainder of pi/2; sin r = r + r^3 (S1 + r^2 (S2 + r^2 (S3 + r^2 S4))),
+ * cos r = (1 - r^2/2) + r^4 (C1 + r^2 (C2 + r^2 (C3 + r^2 C4))), coefficients
+ * fitted to the Taylor remainders and rounded to float; then the quadrant
+ * k mod 4 (computed in float: NaN and inf land in quadrant 0). */
+static void ho_sincosf(float x, float* sn, float* cs)
+{
+    const float k = floorf(x * 0.636619772367581343f + 0.5f);
+    const float r = ((x - k * 1.5703125f) - k * 4.84466552734375e-4f) - k * -6.397578431460715e-7f;
+    const float r2 = r * r;
+    float p = 2.7181240511708893e-6f;
+    p = p * r2 + -1.9839312881231308e-4f;
+    p = p * r2 + 8.333329111337662e-3f;
+    p = p * r2 + -0.1666666716337204f;
+    const float s = r + (r * r2) * p;
+    float q = -2.7207511266169604e-7f;
+    q = q * r2 + 2.4799483071547e-5f;
+    q = q * r2 + -1.3888883404433727e-3f;
+    q = q * r2 + 4.16666679084301e-2f;
+    const float c = (1.0f - 0.5f * r2) + (r2 * r2) * q;
+    const float m = k - 4.0f * floorf(k * 0.25f);
+    if (m == 1.0f) { *sn = c; *cs = -s; }
+    else if (m == 2.0f) { *sn = -s; *cs = -c; }
+    else if (m == 3.0f) { *sn = -c; *cs = s; }
+    else { *sn = s; *cs = c; }
+}
+
+float ho_sinf(float x) { float s, c; ho_sincosf(x, &s, &c); return s; }
+float ho_cosf(float x) { float s, c; ho_sincosf(x, &s, &c); return c; }
+
 #define R float
 #define SFX _f32
 #define SQRT sqrtf
+#define SIN ho_sinf
+#define COS ho_cosf
 #define FLOOR floorf
 #define FABS fabsf
 #define FMIN fminf
@@ -29,14 +64,20 @@
 #undef R
 #undef SFX
 #undef SQRT
+#undef SIN
+#undef COS
 #undef FLOOR
 #undef FABS
 #undef FMIN
 #undef FMAX
 
+float ho_fbmf(float x, float y) { return fbm_f32(x, y); }
+
 #define R double
 #define SFX _f64
 #define SQRT sqrt
+#define SIN sin
+#define COS cos
 #define FLOOR floor
 #define FABS fabs
 #define FMIN fmin

@@ -65,6 +65,10 @@ typedef struct ho_params {
     const float* mesh_tris;
     int mesh_ntris;
     float mesh_margin;
+    /* Wind forcing carried commented out in the shader (compute.comp:150-151):
+     * 0 none, 1 line 150, 2 line 151 (fbm-modulated), at Time.totalTime. */
+    float total_time;
+    int forcing;
 } ho_params;
 
 /* Mesh collider.  The reference has no mesh collider (compute.comp:155-184
@@ -108,6 +112,14 @@ int ho_force_f64(const ho_params* p, const double pos[3], double noise, double o
 
 /* Saturating float->int32 (the int() of the contract above). */
 int32_t ho_sat_i32_f32(float f);
+/* binary32 sin / cos / fbm of the forcing: one fixed algorithm (range
+ * reduction k = floor(x*2/pi + 0.5), three-part Cody-Waite split of pi/2,
+ * degree-9 odd and degree-10 even polynomials on [-pi/4, pi/4]) whose every
+ * operation is a single rounded binary32 add, sub or mul, so any IEEE
+ * implementation of the same steps returns the same bits. */
+float ho_sinf(float x);
+float ho_cosf(float x);
+float ho_fbmf(float x, float y);
 int32_t ho_sat_i32_f64(double f);
 
 /* Multithreaded FP32 step (OpenMP over strands, same results as ho_step_f32);

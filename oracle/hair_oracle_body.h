@@ -206,14 +206,64 @@ static int F(mesh_contact)(const ho_params* p, V3 x, R k, V3* acc)
     return 1;
 }
 
+/* random / noise / fbm of compute.comp:81-120 (mix(a,b,t) = a*(1-t) + b*t,
+ * fract(x) = x - floor(x)). */
+static inline R F(fract)(R x) { return x - FLOOR(x); }
+static inline R F(random)(R px, R py)
+{
+    return F(fract)(SIN(px * (R)12.9898f + py * (R)78.233f) * (R)43758.5453123f);
+}
+static R F(noise2)(R px, R py)
+{
+    const R ix = FLOOR(px), iy = FLOOR(py);
+    const R fx = F(fract)(px), fy = F(fract)(py);
+    const R a = F(random)(ix, iy);
+    const R b = F(random)(ix + (R)1, iy + (R)0);
+    const R c = F(random)(ix + (R)0, iy + (R)1);
+    const R d = F(random)(ix + (R)1, iy + (R)1);
+    const R ux = (fx * fx) * ((R)3 - (R)2 * fx);
+    const R uy = (fy * fy) * ((R)3 - (R)2 * fy);
+    const R mix = a * ((R)1 - ux) + b * ux;
+    return (mix + ((c - a) * uy) * ((R)1 - ux)) + ((d - b) * ux) * uy;
+}
+static R F(fbm)(R px, R py)
+{
+    R value = 0, amplitude = (R)0.5;
+    for (int i = 0; i < 6; ++i) {
+        value = value + amplitude * F(noise2)(px, py);
+        px = px * (R)2;
+        py = py * (R)2;
+        amplitude = amplitude * (R)0.5;
+    }
+    return value;
+}
+
+/* The forcing of compute.comp:150 (mode 1) or :151 (mode 2) at position x and
+ * Time.totalTime t, the scalar factor on the left as the shader writes it:
+ *   1: 10.0 * vec3(2.0*sin(t*2.0)*cos(y*10.0)*sin((y+5.0)*15.0), 0.0, -clamp(y*2.0, 0.2, 2.0))
+ *   2: 7.0*fbm(vec2(sin(t), cos(t))) *
+ *        vec3(2.0*sin(t*2.0)*cos(y*10.0)*sin((y+5.0)*15.0), 4.0*sin(z*5.0 + t*3.0), -0.6*(y+3.0)) */
+static V3 F(forcing)(const ho_params* p, V3 x)
+{
+    const R t = (R)p->total_time;
+    const R sx = (((R)2 * SIN(t * (R)2)) * COS(x.y * (R)10)) * SIN((x.y + (R)5) * (R)15);
+    if (p->forcing == 1) {
+        const R cl = FMIN(FMAX(x.y * (R)2, (R)0.2f), (R)2);
+        return F(smul)((R)10, F(v3)(sx, (R)0, -cl));
+    }
+    const R amp = (R)7 * F(fbm)(SIN(t), COS(t));
+    return F(smul)(amp, F(v3)(sx, (R)4 * SIN(x.z * (R)5 + t * (R)3), (R)-0.6f * (x.y + (R)3)));
+}
+
 /* Force at a start-of-step position: gravity (+ wind, + per-strand noise along
- * noise_dir) then the collider penalty loop, compute.comp:150-184. */
+ * noise_dir, + the forcing) then the collider penalty loop, compute.comp:150-184. */
 static int F(force)(const ho_params* p, V3 x, R noise, V3* out)
 {
     V3 f = F(vadd)(F(v3)((R)p->gravity[0], (R)p->gravity[1], (R)p->gravity[2]),
                    F(v3)((R)p->wind[0], (R)p->wind[1], (R)p->wind[2]));
     if (p->wind_noise)
         f = F(vadd)(f, F(smul)(noise, F(v3)((R)p->noise_dir[0], (R)p->noise_dir[1], (R)p->noise_dir[2])));
+    if (p->forcing) f = F(vadd)(f, F(forcing)(p, x));
     const R k = (R)p->spring_k;
     V3 acc = F(v3)(0, 0, 0);
     int hits = 0;

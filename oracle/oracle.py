@@ -39,6 +39,8 @@ class HoParams(C.Structure):
         ("mesh_tris", C.POINTER(C.c_float)),
         ("mesh_ntris", C.c_int),
         ("mesh_margin", C.c_float),
+        ("total_time", C.c_float),
+        ("forcing", C.c_int),
     ]
 
 MESH_TRI_FLOATS = 40
@@ -78,6 +80,11 @@ def lib():
         L.ho_sat_i32_f32.restype = C.c_int32
         L.ho_sat_i32_f64.argtypes = [C.c_double]
         L.ho_sat_i32_f64.restype = C.c_int32
+        for fn in ("ho_sinf", "ho_cosf"):
+            getattr(L, fn).argtypes = [C.c_float]
+            getattr(L, fn).restype = C.c_float
+        L.ho_fbmf.argtypes = [C.c_float, C.c_float]
+        L.ho_fbmf.restype = C.c_float
         _lib = L
     return _lib
 
@@ -88,7 +95,7 @@ class Params:
     def __init__(self, *, dt, gravity=(0.0, -9.8, 0.0), wind=(0.0, 0.0, 0.0), damping=0.998, spring_k=1900.0,
                  sphere_radius=1.0, vmax=10.0, friction=0.08, hair_hair=False, grid_dim=0,
                  grid_origin=(0.0, 0.0, 0.0), grid_h=1.0, colliders=None, seg_length=None, wind_noise=None,
-                 noise_dir=(1.0, 0.0, 0.0), mesh=None, mesh_margin=0.0):
+                 noise_dir=(1.0, 0.0, 0.0), mesh=None, mesh_margin=0.0, total_time=0.0, forcing=0):
         p = HoParams()
         p.dt = dt
         p.gravity = (C.c_float * 3)(*gravity)
@@ -116,6 +123,8 @@ class Params:
         p.mesh_tris = None if self._mesh is None else self._mesh.ctypes.data_as(C.POINTER(C.c_float))
         p.mesh_ntris = 0 if self._mesh is None else self._mesh.shape[0]
         p.mesh_margin = mesh_margin
+        p.total_time = total_time
+        p.forcing = forcing
         self.p = p
 
     def grid_buffer(self):

@@ -21,6 +21,7 @@
 #include "hs_kernels.h"
 
 #include "hs_internal.h"
+#include "hs_forcing.hpp"
 #include "hs_mesh_build.h"
 
 using namespace hsi;
@@ -302,6 +303,7 @@ int build_args(hs_ctx* c, Groom& g, const hs_time* t, const hs_collider* cols, u
     if (p->num_colliders > ncol) return fail(c, HS_ERR_INVALID, w + ": params.num_colliders > colliders given");
     if (!(t->deltaTime > 0.0f) || !std::isfinite(t->deltaTime))
         return fail(c, HS_ERR_INVALID, w + ": deltaTime must be finite and > 0");
+    if (p->forcing > 2) return fail(c, HS_ERR_INVALID, w + ": forcing must be 0, 1 or 2");
     const hsi::Mesh* mesh = nullptr;
     if (p->mesh_id != 0) {
         auto it = c->meshes.find(p->mesh_id);
@@ -342,6 +344,14 @@ int build_args(hs_ctx* c, Groom& g, const hs_time* t, const hs_collider* cols, u
     a.ndx = g.noise_dir[0];
     a.ndy = g.noise_dir[1];
     a.ndz = g.noise_dir[2];
+    if (p->forcing) {
+        // per-step constants of the forcing (fbm is uniform over the groom)
+        const hs::Forcing fc = hs::forcing_at(p->forcing, t->totalTime);
+        a.forcing = fc.mode;
+        a.f_amp = fc.amp;
+        a.f_a2 = fc.a2;
+        a.f_t3 = fc.t3;
+    }
     a.ncol = p->num_colliders;
     a.cols = g.cols;
     if (mesh) a.mesh = mesh->args;

@@ -100,6 +100,8 @@ struct StepArgs {
     const Collider* cols; // device copy of all ncol colliders (read for ellipsoids 1..ncol-1)
     MeshArgs mesh;        // triangle-mesh collider (mesh.tris null: none)
     const float4* mesh_f; // [N][Sp] mesh contact of the stored positions (launch_mesh_force), or null
+    uint32_t forcing;     // wind forcing mode (0: none); its per-step constants:
+    float f_amp, f_a2, f_t3;
     // tile choices (results do not depend on them)
     uint32_t win_gather;  // voxel cells of the gather window (<= kWinCells)
     uint32_t win_splat;   // voxel cells of a splat window (<= kWinCells / HS_VOXEL_COPIES)

@@ -31,6 +31,7 @@
 #include <stdint.h>
 
 #include "hs_device_math.hpp"
+#include "hs_forcing.hpp"
 #include "hs_kernels.h"
 
 namespace hs {
@@ -491,6 +492,12 @@ __global__ __launch_bounds__(256) void k_mesh_force(MeshArgs M, const float* __r
 template <int COL>
 __device__ __forceinline__ f3 force_at(f3 x, f3 fs, const StepArgs& a, float4 mf)
 {
+    if (a.forcing) {
+        // the shader's commented-out wind (compute.comp:150-151), after gravity
+        float ft[3];
+        forcing_term(Forcing{a.forcing, a.f_amp, a.f_a2, a.f_t3}, x.y, x.z, ft);
+        fs = add(fs, mk(ft[0], ft[1], ft[2]));
+    }
     f3 acc = mk(0.0f, 0.0f, 0.0f);
     int hits = 0;
     if (a.ncol > 0) {

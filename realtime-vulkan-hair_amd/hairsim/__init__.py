@@ -96,6 +96,7 @@ class HsStepParams(C.Structure):
         ("num_colliders", C.c_uint32),
         ("hair_hair", C.c_uint32),
         ("mesh_id", C.c_uint32),
+        ("forcing", C.c_uint32),
     ]
 
 
@@ -239,7 +240,8 @@ def reference_step_params():
 
 
 def step_params(gravity=(0.0, -9.8, 0.0), wind=(0.0, 0.0, 0.0), damping=0.998, spring_k=1900.0,
-                sphere_radius=1.0, vmax=10.0, friction=0.08, num_colliders=0, hair_hair=True, mesh_id=0):
+                sphere_radius=1.0, vmax=10.0, friction=0.08, num_colliders=0, hair_hair=True, mesh_id=0,
+                forcing=0):
     p = HsStepParams()
     p.gravity = _f3(gravity)
     p.wind = _f3(wind)
@@ -251,6 +253,7 @@ def step_params(gravity=(0.0, -9.8, 0.0), wind=(0.0, 0.0, 0.0), damping=0.998, s
     p.num_colliders = num_colliders
     p.hair_hair = 1 if hair_hair else 0
     p.mesh_id = mesh_id
+    p.forcing = forcing
     return p
 
 

@@ -458,3 +458,99 @@ def test_ref32_tracks_oracle64_per_step():
         P64 = s64.reshape(-1, 3, g.N, 4)[:, 0, :, :3]
         worst = max(worst, float(np.max(np.abs(P32 - P64) / np.maximum(np.abs(P64), 1.0))))
     assert worst <= 1e-5, worst
+
+
+# ------------------------------------------- forcing (compute.comp:150-151) --
+def _trig_samples(lo, hi, n, seed):
+    return np.random.default_rng(seed).uniform(lo, hi, n).astype(np.float32)
+
+
+@pytest.mark.parametrize("lo,hi,abs_tol,ulp_tol", [
+    (-math.pi / 4, math.pi / 4, 8e-8, 1.5), (-10.0, 10.0, 1e-7, 2.0), (-1e5, 1e5, 1.2e-7, None)])
+def test_forcing_sin_cos_against_binary64(lo, hi, abs_tol, ulp_tol):
+    """The oracle's binary32 sin/cos against libm's binary64 ones: absolute
+    error below 1.2e-7 everywhere up to 1e5, within 2 ulp near the origin."""
+    L = oracle.lib()
+    x = _trig_samples(lo, hi, 20000, 7)
+    for f, ref in ((L.ho_sinf, np.sin), (L.ho_cosf, np.cos)):
+        got = np.array([f(float(v)) for v in x], dtype=np.float64)
+        want = ref(x.astype(np.float64))
+        err = np.abs(got - want)
+        assert err.max() < abs_tol, (f.__name__, err.max())
+        if ulp_tol is not None:
+            ulp = np.spacing(np.abs(want).astype(np.float32)).astype(np.float64)
+            assert (err / ulp).max() <= ulp_tol, (f.__name__, (err / ulp).max())
+
+
+def test_forcing_sin_cos_special_values():
+    L = oracle.lib()
+    assert L.ho_sinf(0.0) == 0.0 and L.ho_cosf(0.0) == 1.0
+    assert math.isnan(L.ho_sinf(float("nan"))) and math.isnan(L.ho_cosf(float("inf")))
+    # quadrant logic: sin(x + pi/2) = cos(x), sin(x + pi) = -sin(x), within
+    # the rounding of the shifted argument to binary32
+    for x in (0.3, -1.1, 2.0, 40.0):
+        tol = 3e-7 + float(np.spacing(np.float32(abs(x) + 4.0)))
+        assert abs(L.ho_sinf(x + math.pi / 2) - L.ho_cosf(x)) < tol
+        assert abs(L.ho_sinf(x + math.pi) + L.ho_sinf(x)) < tol
+
+
+def test_fbm_on_the_lattice_reduces_to_the_hash():
+    """At integer points every octave sits on a lattice node, where noise()
+    is the corner hash itself (u = 0): fbm = sum 2^-(i+1) random(2^i p)."""
+    L = oracle.lib()
+    f32 = np.float32
+    for px, py in ((0.0, 0.0), (3.0, -2.0), (-7.0, 5.0), (1.0, 1.0)):
+        want = f32(0.0)
+        amp = f32(0.5)
+        qx, qy = f32(px), f32(py)
+        for _ in range(6):
+            d = qx * f32(12.9898) + qy * f32(78.233)
+            h = f32(L.ho_sinf(float(d))) * f32(43758.5453123)
+            want = want + amp * (h - np.floor(h))
+            qx, qy, amp = qx * f32(2), qy * f32(2), amp * f32(0.5)
+        assert L.ho_fbmf(px, py) == want
+
+
+def test_fbm_is_continuous_and_bounded():
+    """The Hermite blend makes noise continuous across cell edges; a wrong
+    corner or blend term would jump there.  Six octaves of [0,1) noise with
+    amplitudes 1/2 .. 1/64 stay in [0, 63/64)."""
+    L = oracle.lib()
+    rng = np.random.default_rng(11)
+    for _ in range(200):
+        n = float(rng.integers(-20, 20))
+        y = float(rng.uniform(-20, 20))
+        for a, b in (((n - 1e-4, y), (n + 1e-4, y)), ((y, n - 1e-4), (y, n + 1e-4))):
+            assert abs(L.ho_fbmf(*a) - L.ho_fbmf(*b)) < 0.01
+    vals = [L.ho_fbmf(float(x), float(y)) for x, y in rng.uniform(-50, 50, (2000, 2))]
+    assert min(vals) >= 0.0 and max(vals) < 63.0 / 64.0
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_forcing_force_closed_form(mode):
+    """Force with the forcing on, against the shader lines evaluated in
+    binary64 (with the oracle's fbm value for mode 2): gravity plus the line,
+    no colliders."""
+    L = oracle.lib()
+    rng = np.random.default_rng(20 + mode)
+    for _ in range(50):
+        t = float(rng.uniform(0, 30))
+        x = rng.uniform(-3, 3, 3).astype(np.float32)
+        p = oracle.Params(dt=1 / 60, total_time=t, forcing=mode)
+        hits, f = oracle.force32(p, x)
+        assert hits == 0
+        X, Y, Z = (float(v) for v in x)
+        tf = float(np.float32(t))
+        sx = 2.0 * math.sin(tf * 2.0) * math.cos(Y * 10.0) * math.sin((Y + 5.0) * 15.0)
+        if mode == 1:
+            want = np.array([10 * sx, -9.8, -10 * min(max(Y * 2.0, 0.2), 2.0)])
+        else:
+            amp = 7.0 * L.ho_fbmf(L.ho_sinf(tf), L.ho_cosf(tf))
+            want = np.array([amp * sx, -9.8 + amp * 4.0 * math.sin(Z * 5.0 + tf * 3.0), amp * -0.6 * (Y + 3.0)])
+        np.testing.assert_allclose(f, want, rtol=2e-5, atol=2e-4)
+
+
+def test_forcing_off_is_the_reference_force():
+    x = np.array([0.4, 1.3, -0.2], dtype=np.float32)
+    a = oracle.force32(oracle.Params(dt=1 / 60, total_time=3.0, forcing=0), x)[1]
+    assert np.array_equal(a, np.array([0.0, -9.8, 0.0], dtype=np.float32))
