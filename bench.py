@@ -436,13 +436,61 @@ def bench_served(args, hairsim, scenes):
     print(json.dumps(res), flush=True)
 
 
+def bench_export(args, hairsim, scenes, samples=64):
+    """Render-facing output of the 100k x 32 groom: hs_export_strands (Strand
+    AoS + indirect record into device memory) and hs_resample_strands (the
+    tessellation curve, `samples` points per strand), each timed over
+    args.steps back-to-back calls between two stream synchronisations."""
+    wl = scenes.plain_100k()
+    g = wl.grooms[0]
+    ctx = hairsim.Context(0)
+    groom = ctx.groom(g.strands, g.N, seg_length=g.seg, wind_noise=g.noise, noise_dir=g.noise_dir,
+                      grid_dim=g.grid_dim, grid_origin=g.grid_origin, grid_h=g.grid_h)
+    for k in range(4):  # pending grid friction: the export applies it
+        groom.step(wl.dt, float(wl.time(k)), [scenes.sphere_collider(wl.sphere(k))], make_params(hairsim, wl, k))
+    aos = hairsim.DeviceBuffer(48 * g.S * g.N)
+    ind = hairsim.DeviceBuffer(16)
+    crv = hairsim.DeviceBuffer(16 * g.S * samples)
+    res = {}
+    for name, fn, nbytes in (
+            # export: pos + vel planes read, grid cells of the friction gather, AoS written
+            ("hs_export_strands", lambda: groom.export(aos, ind), 24 * g.S * g.N + 48 * g.S * g.N),
+            # resample: positions read once, samples written
+            ("hs_resample_strands", lambda: groom.resample(samples, crv), 12 * g.S * g.N + 16 * g.S * samples)):
+        for _ in range(args.warmup):
+            fn()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        ctx.sync()
+        us = (time.perf_counter() - t0) / args.steps * 1e6
+        gbs = nbytes / (us * 1e-6) / 1e9
+        res[name] = {"us_per_call": us, "algorithmic_bytes": nbytes, "achieved": gbs, "frac": gbs / PEAK_HBM_GBS}
+    for b in (aos, ind, crv):
+        b.free()
+    groom.destroy()
+    ctx.close()
+    e = res["hs_export_strands"]
+    print(json.dumps({
+        "metric": "strand-point exports/sec", "value": g.S * g.N / (e["us_per_call"] * 1e-6),
+        "unit": "points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": e["us_per_call"] / 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"export-100k: {g.S} strands x {g.N} points to the Strand layout + "
+                               f"{samples} curve samples per strand, device memory"},
+        "roofline": {"bound": "hbm", "achieved": e["achieved"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": e["frac"], "traffic": None, "kernel": "k_unpack"},
+        "calls": res}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=240)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="plain-100k",
-                    choices=["plain-100k", "large-1m", "plain-1k", "batched-256", "served"])
+                    choices=["plain-100k", "large-1m", "plain-1k", "batched-256", "served", "export-100k"])
     ap.add_argument("--rate", type=float, default=2000.0, help="served: job arrivals per second")
     ap.add_argument("--seconds", type=float, default=10.0, help="served: arrival window")
     ap.add_argument("--max-points", type=int, default=12_000_000, help="served: points per batched launch")
@@ -468,6 +516,9 @@ def main():
         return
     if args.workload == "served":
         bench_served(args, hairsim, scenes)
+        return
+    if args.workload == "export-100k":
+        bench_export(args, hairsim, scenes)
         return
     wl = {"plain-100k": scenes.plain_100k, "large-1m": scenes.large_1m, "plain-1k": scenes.plain_1k}[args.workload]()
     bench_single(args, hairsim, scenes, wl)

@@ -240,6 +240,22 @@ int hs_autotune_batch(hs_ctx* ctx, const hs_batch_item* items, uint32_t num_item
  * Either output may be NULL.  Blocks. */
 int hs_read_strands(hs_ctx* ctx, uint32_t groom_id, float* strands_aos,
                     hs_draw_indirect* indirect);
+/* Render-facing export: the strand SSBO the reference renderer binds as its
+ * vertex buffer and the indirect draw record it draws with (Strand.h:11-58,
+ * Renderer.cpp:2240-2254, 2295), written into DEVICE memory: the groom in the
+ * Strand layout (pending grid friction applied) into `strands_dev` (48*S*N
+ * bytes) and, unless NULL, {S, 1, 0, 0} into `indirect_dev` (16 B).  Enqueued
+ * on the library stream (hs_sync waits); nothing crosses the host link. */
+int hs_export_strands(hs_ctx* ctx, uint32_t groom_id, float* strands_dev, uint32_t* indirect_dev);
+/* The curve the hair tessellation shader draws through each strand
+ * (hair.tese:34-80, `func`, one guide strand): M >= 2 samples per strand at
+ * v = j/(M-1) on the piecewise cubic Bezier with control points
+ * b1 = v1 + (1/3)((v2 - v0)/2), b2 = v2 - (1/3)((v3 - v1)/2), end neighbours
+ * mirrored, de Casteljau with mix(a,b,t) = a(1-t) + bt; v = 1 is the tip
+ * (where the shader would read one point past the strand).  Positions as
+ * float4 (w = 1), S*M*16 bytes of DEVICE memory `out_dev`, caller strand
+ * order.  Enqueued on the library stream. */
+int hs_resample_strands(hs_ctx* ctx, uint32_t groom_id, uint32_t samples, float* out_dev);
 /* Overwrite the groom's state from a reference-layout buffer (checkpoint
  * restore; clears any pending grid friction). */
 int hs_write_strands(hs_ctx* ctx, uint32_t groom_id, const float* strands_aos);

@@ -48,6 +48,56 @@ static void ho_sincosf(float x, float* sn, float* cs)
     else { *sn = s; *cs = c; }
 }
 
+/* mix(a, b, t) = a*(1-t) + b*t per component (GLSL) */
+static void ho_mix3(const float* a, const float* b, float t, float* o)
+{
+    const float u = 1.0f - t;
+    for (int k = 0; k < 3; ++k) o[k] = a[k] * u + b[k] * t;
+}
+
+void ho_resample_f32(const float* strands, int S, int N, int M, float* out)
+{
+    const float third = 1.0f / 3.0f;
+    for (int s = 0; s < S; ++s) {
+        const float* P = strands + (size_t)s * 12 * N; /* curvePoints[N] (vec4) */
+        for (int j = 0; j < M; ++j) {
+            const float v = (float)j / (float)(M - 1);
+            const float x = v * (float)(N - 1);
+            const float fl = floorf(x);
+            int seg = (int)fl;
+            float t = x - fl;
+            if (seg >= N - 1) { seg = N - 2; t = 1.0f; }
+            float v0[3], v1[3], v2[3], v3[3], b0[3], b1[3], b2[3], b3[3];
+            for (int k = 0; k < 3; ++k) {
+                v1[k] = P[4 * seg + k];
+                v2[k] = P[4 * (seg + 1) + k];
+            }
+            for (int k = 0; k < 3; ++k) {
+                v0[k] = seg == 0 ? v1[k] + (v1[k] - v2[k]) : P[4 * (seg - 1) + k];
+                v3[k] = seg + 2 == N ? v2[k] + (v2[k] - v1[k]) : P[4 * (seg + 2) + k];
+            }
+            for (int k = 0; k < 3; ++k) {
+                b0[k] = v1[k];
+                b1[k] = v1[k] + third * ((v2[k] - v0[k]) / 2.0f);
+                b2[k] = v2[k] - third * ((v3[k] - v1[k]) / 2.0f);
+                b3[k] = v2[k];
+            }
+            float b01[3], b11[3], b21[3], b02[3], b12[3], c[3];
+            ho_mix3(b0, b1, t, b01);
+            ho_mix3(b1, b2, t, b11);
+            ho_mix3(b2, b3, t, b21);
+            ho_mix3(b01, b11, t, b02);
+            ho_mix3(b11, b21, t, b12);
+            ho_mix3(b02, b12, t, c);
+            float* o = out + ((size_t)s * M + j) * 4;
+            o[0] = c[0];
+            o[1] = c[1];
+            o[2] = c[2];
+            o[3] = 1.0f;
+        }
+    }
+}
+
 float ho_sinf(float x) { float s, c; ho_sincosf(x, &s, &c); return s; }
 float ho_cosf(float x) { float s, c; ho_sincosf(x, &s, &c); return c; }
 

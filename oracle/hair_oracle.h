@@ -120,6 +120,12 @@ int32_t ho_sat_i32_f32(float f);
 float ho_sinf(float x);
 float ho_cosf(float x);
 float ho_fbmf(float x, float y);
+/* The curve of the hair tessellation shader (hair.tese:34-80, `func`, one
+ * guide strand) through each strand of the AoS buffer `strands` (S strands of
+ * N points, Strand layout): M samples per strand at v = j/(M-1), written as
+ * xyz1 into out[(s*M + j)*4 ..].  v = 1 evaluates the last segment at t = 1
+ * (the shader's index there runs one past the strand). */
+void ho_resample_f32(const float* strands, int S, int N, int M, float* out);
 int32_t ho_sat_i32_f64(double f);
 
 /* Multithreaded FP32 step (OpenMP over strands, same results as ho_step_f32);

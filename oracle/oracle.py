@@ -85,6 +85,7 @@ def lib():
             getattr(L, fn).restype = C.c_float
         L.ho_fbmf.argtypes = [C.c_float, C.c_float]
         L.ho_fbmf.restype = C.c_float
+        L.ho_resample_f32.argtypes = [FP, C.c_int, C.c_int, C.c_int, FP]
         _lib = L
     return _lib
 
@@ -210,6 +211,16 @@ def force64(params, pos, noise=0.0):
     DP = C.POINTER(C.c_double)
     hits = lib().ho_force_f64(C.byref(params.p), pos.ctypes.data_as(DP), noise, out.ctypes.data_as(DP))
     return hits, out
+
+
+def resample32(strands, N, M):
+    """Tessellation-shader curve samples: (S, M, 4) float32."""
+    a = np.ascontiguousarray(strands, dtype=np.float32).reshape(-1)
+    S = a.size // (12 * N)
+    out = np.zeros((S, M, 4), dtype=np.float32)
+    FP = C.POINTER(C.c_float)
+    lib().ho_resample_f32(a.ctypes.data_as(FP), S, N, M, out.ctypes.data_as(FP))
+    return out
 
 
 def sat_i32(f):

@@ -1239,6 +1239,31 @@ int hs_read_strands(hs_ctx* c, uint32_t id, float* aos, hs_draw_indirect* ind)
     return HS_OK;
 }
 
+int hs_export_strands(hs_ctx* c, uint32_t id, float* aos_dev, uint32_t* ind_dev)
+{
+    if (!c) return HS_ERR_INVALID;
+    if (!aos_dev) return fail(c, HS_ERR_INVALID, "hs_export_strands: null buffer");
+    Groom* g = find(c, id);
+    if (!g) return fail(c, HS_ERR_NOT_FOUND, "hs_export_strands: unknown groom");
+    HS_TRY(c, hipSetDevice(c->device));
+    HS_TRY(c, groom_unpack(*g, aos_dev, c->stream));
+    if (ind_dev) HS_TRY(c, hs::launch_indirect(ind_dev, g->S, c->stream));
+    return HS_OK;
+}
+
+int hs_resample_strands(hs_ctx* c, uint32_t id, uint32_t M, float* out_dev)
+{
+    if (!c) return HS_ERR_INVALID;
+    if (!out_dev) return fail(c, HS_ERR_INVALID, "hs_resample_strands: null buffer");
+    if (M < 2) return fail(c, HS_ERR_INVALID, "hs_resample_strands: samples must be >= 2");
+    Groom* g = find(c, id);
+    if (!g) return fail(c, HS_ERR_NOT_FOUND, "hs_resample_strands: unknown groom");
+    if ((uint64_t)g->S * M >= (1ull << 40)) return fail(c, HS_ERR_INVALID, "hs_resample_strands: too many samples");
+    HS_TRY(c, hipSetDevice(c->device));
+    HS_TRY(c, hs::launch_resample(g->pos, g->perm, g->S, g->N, g->L.Sp, M, out_dev, c->stream));
+    return HS_OK;
+}
+
 int hs_write_strands(hs_ctx* c, uint32_t id, const float* aos)
 {
     if (!c) return HS_ERR_INVALID;

@@ -131,6 +131,11 @@ hipError_t launch_reduce_batch(const StepArgs* args_dev, const uint32_t* brick_s
                                uint32_t total_bricks, hipStream_t st);
 hipError_t launch_pack(const float* aos_dev, const uint32_t* perm, uint32_t S, uint32_t N, uint32_t Sp,
                        float* pos, float* vel, float* corr, hipStream_t st);
+// Tessellation-shader curve samples (M per strand, float4 w = 1) in caller
+// strand order; the indirect draw record {S, 1, 0, 0}.
+hipError_t launch_resample(const float* pos, const uint32_t* perm, uint32_t S, uint32_t N, uint32_t Sp, uint32_t M,
+                           float* out_dev, hipStream_t st);
+hipError_t launch_indirect(uint32_t* rec_dev, uint32_t S, hipStream_t st);
 hipError_t launch_unpack(float* aos_dev, const uint32_t* perm, uint32_t S, uint32_t N, uint32_t Sp,
                          const float* pos, const float* vel, const float* corr, const UnpackGrid* g,
                          hipStream_t st);

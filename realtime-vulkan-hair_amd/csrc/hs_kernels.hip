@@ -1329,6 +1329,63 @@ __global__ void k_unpack(float4* __restrict__ aos, const uint32_t* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// Render-facing output
+// ---------------------------------------------------------------------------
+// Curve of the hair tessellation shader (hair.tese:34-80, `func`, one strand):
+// sample j of M at v = j/(M-1); segment floor(v*(N-1)) between points v1, v2
+// with neighbours v0, v3 (mirrored at the ends: v1 + (v1 - v2), v2 + (v2 -
+// v1)); Bezier control points b1 = v1 + (1/3)((v2 - v0)/2), b2 = v2 -
+// (1/3)((v3 - v1)/2); de Casteljau with mix(a,b,t) = a*(1-t) + b*t.  At v = 1
+// the shader would read one point past the strand; here it is the tip
+// (last segment, t = 1).  One thread per (strand slot, sample), the sample
+// index fastest so each strand's output is one contiguous run.
+__device__ __forceinline__ f3 mix3(f3 a, f3 b, float t)
+{
+    const float u = 1.0f - t;
+    return mk(a.x * u + b.x * t, a.y * u + b.y * t, a.z * u + b.z * t);
+}
+
+__global__ void k_resample(const float* __restrict__ pos, const uint32_t* __restrict__ perm, uint32_t S, uint32_t N,
+                           uint32_t Sp, uint32_t M, float4* __restrict__ out)
+{
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (size_t)S * M) return;
+    const uint32_t p = (uint32_t)(tid / M), j = (uint32_t)(tid % M);
+    const size_t plane = (size_t)N * Sp;
+    auto P = [&](uint32_t i) {
+        const size_t o = (size_t)i * Sp + p;
+        return mk(pos[o], pos[plane + o], pos[2 * plane + o]);
+    };
+    const float v = (float)j / (float)(M - 1);
+    const float x = v * (float)(N - 1);
+    const float fl = floorf(x);
+    uint32_t seg = (uint32_t)fl;
+    float t = x - fl;
+    if (seg >= N - 1) {
+        seg = N - 2;
+        t = 1.0f;
+    }
+    const f3 v1 = P(seg), v2 = P(seg + 1);
+    const f3 v0 = seg == 0 ? add(v1, sub(v1, v2)) : P(seg - 1);
+    const f3 v3 = seg + 2 == N ? add(v2, sub(v2, v1)) : P(seg + 2);
+    const float third = 1.0f / 3.0f;
+    const f3 b0 = v1;
+    const f3 b1 = add(v1, smul(third, muls(sub(v2, v0), 0.5f)));
+    const f3 b2 = sub(v2, smul(third, muls(sub(v3, v1), 0.5f)));
+    const f3 b3 = v2;
+    const f3 b01 = mix3(b0, b1, t), b11 = mix3(b1, b2, t), b21 = mix3(b2, b3, t);
+    const f3 b02 = mix3(b01, b11, t), b12 = mix3(b11, b21, t);
+    const f3 c = mix3(b02, b12, t);
+    out[(size_t)perm[p] * M + j] = make_float4(c.x, c.y, c.z, 1.0f);
+}
+
+// The indirect draw record the shader's count produces (compute.comp:302).
+__global__ void k_indirect(uint32_t* __restrict__ rec, uint32_t S)
+{
+    if (threadIdx.x < 4) rec[threadIdx.x] = threadIdx.x == 0 ? S : (threadIdx.x == 1 ? 1u : 0u);
+}
+
+// ---------------------------------------------------------------------------
 // Host-side launchers
 // ---------------------------------------------------------------------------
 template <bool G, bool S>
@@ -1443,6 +1500,22 @@ hipError_t launch_pack(const float* aos_dev, const uint32_t* perm, uint32_t S, u
     const size_t blocks = (n + bs - 1) / bs;
     hipLaunchKernelGGL(k_pack, dim3((uint32_t)blocks), dim3(bs), 0, st, reinterpret_cast<const float4*>(aos_dev),
                        perm, S, N, Sp, pos, vel, corr);
+    return hipGetLastError();
+}
+
+hipError_t launch_resample(const float* pos, const uint32_t* perm, uint32_t S, uint32_t N, uint32_t Sp, uint32_t M,
+                           float* out_dev, hipStream_t st)
+{
+    if (N < 2 || M < 2) return hipErrorInvalidValue;
+    const size_t n = (size_t)S * M;
+    hipLaunchKernelGGL(k_resample, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, pos, perm, S, N, Sp, M,
+                       reinterpret_cast<float4*>(out_dev));
+    return hipGetLastError();
+}
+
+hipError_t launch_indirect(uint32_t* rec_dev, uint32_t S, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_indirect, dim3(1), dim3(64), 0, st, rec_dev, S);
     return hipGetLastError();
 }
 

@@ -185,6 +185,8 @@ _SIGS = {
     "hs_profile_read": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
     "hs_profile_breakdown": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "hs_profile_mesh": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "hs_export_strands": (C.c_int, [_P, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "hs_resample_strands": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_void_p]),
     "hs_collider_from_trs": (None, [C.POINTER(HsCollider), C.POINTER(C.c_float), C.POINTER(C.c_float),
                                     C.POINTER(C.c_float)]),
     "hs_collider_translate": (None, [C.POINTER(HsCollider), C.POINTER(C.c_float)]),
@@ -467,6 +469,58 @@ class Batch:
         self.ctx._check(lib().hs_step_batch(self.ctx._h, self._items, self.n))
 
 
+_hip = None
+
+
+def _hip_runtime():
+    """The HIP runtime libhairsim.so is linked against (already in the process)."""
+    global _hip
+    if _hip is None:
+        lib()
+        H = C.CDLL("libamdhip64.so.7")
+        H.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        H.hipFree.argtypes = [C.c_void_p]
+        H.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        H.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+        _hip = H
+    return _hip
+
+
+class DeviceBuffer:
+    """Device memory for the device-output entry points (hs_export_strands,
+    hs_resample_strands), allocated with the library's own HIP runtime."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        rc = _hip_runtime().hipMalloc(C.byref(p), max(1, self.nbytes))
+        if rc != 0:
+            raise HairSimError(HS_ERR_OUT_OF_MEMORY, f"hipMalloc({self.nbytes}) failed: {rc}")
+        self.ptr = p.value
+
+    def fill(self, byte):
+        _hip_runtime().hipMemset(self.ptr, int(byte), self.nbytes)
+
+    def numpy(self, dtype):
+        """Blocking copy to the host."""
+        out = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype=dtype)
+        rc = _hip_runtime().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2)  # device to host
+        if rc != 0:
+            raise HairSimError(HS_ERR_HIP, f"hipMemcpy failed: {rc}")
+        return out
+
+    def free(self):
+        if self.ptr:
+            _hip_runtime().hipFree(self.ptr)
+            self.ptr = None
+
+
+def _dev(buf):
+    if buf is None:
+        return None
+    return buf.ptr if isinstance(buf, DeviceBuffer) else int(buf)
+
+
 class Groom:
     def __init__(self, ctx, strands_aos, num_points, *, strand_length=2.5, seg_length=None, wind_noise=None,
                  noise_dir=(1.0, 0.0, 0.0), grid_dim=0, grid_origin=(0.0, 0.0, 0.0), grid_h=1.0,
@@ -511,6 +565,15 @@ class Groom:
         ind = HsDrawIndirect()
         self.ctx._check(lib().hs_read_strands(self.ctx._h, self.id, _fptr(out), C.byref(ind)))
         return out, (ind.vertexCount, ind.instanceCount, ind.firstVertex, ind.firstInstance)
+
+    def export(self, strands_dev, indirect_dev=None):
+        """Reference-layout strands (and the indirect record) into device memory
+        (DeviceBuffer or a raw device address); enqueued, ctx.sync() waits."""
+        self.ctx._check(lib().hs_export_strands(self.ctx._h, self.id, _dev(strands_dev), _dev(indirect_dev)))
+
+    def resample(self, samples, out_dev):
+        """Tessellation-shader curve samples (S x samples float4) into device memory."""
+        self.ctx._check(lib().hs_resample_strands(self.ctx._h, self.id, int(samples), _dev(out_dev)))
 
     def write(self, strands_aos):
         aos = np.ascontiguousarray(strands_aos, dtype=np.float32)

@@ -554,3 +554,68 @@ def test_forcing_off_is_the_reference_force():
     x = np.array([0.4, 1.3, -0.2], dtype=np.float32)
     a = oracle.force32(oracle.Params(dt=1 / 60, total_time=3.0, forcing=0), x)[1]
     assert np.array_equal(a, np.array([0.0, -9.8, 0.0], dtype=np.float32))
+
+
+# ------------------------------- curve resampling (hair.tese:34-80, func) --
+def _strands_from_points(P):
+    """(S, N, 3) positions -> Strand AoS (velocities, corrections zero)."""
+    S, N, _ = P.shape
+    a = np.zeros((S, 3, N, 4), dtype=np.float32)
+    a[:, 0, :, :3] = P
+    a[:, 0, :, 3] = 1.0
+    return a.reshape(-1)
+
+
+def _catmull_rom64(P, M):
+    """Independent binary64 Hermite form of the same curve (tangents
+    (v2 - v0)/2 and (v3 - v1)/2, mirrored end neighbours), same sample
+    parameters v = j/(M-1) rounded to binary32."""
+    S, N, _ = P.shape
+    P = P.astype(np.float64)
+    out = np.zeros((S, M, 3))
+    for j in range(M):
+        v = float(np.float32(j) / np.float32(M - 1))
+        x = float(np.float32(v) * np.float32(N - 1))
+        seg = int(math.floor(x))
+        t = x - seg
+        if seg >= N - 1:
+            seg, t = N - 2, 1.0
+        v1, v2 = P[:, seg], P[:, seg + 1]
+        v0 = 2 * v1 - v2 if seg == 0 else P[:, seg - 1]
+        v3 = 2 * v2 - v1 if seg + 2 == N else P[:, seg + 2]
+        m1, m2 = (v2 - v0) / 2, (v3 - v1) / 2
+        h = [2 * t**3 - 3 * t**2 + 1, t**3 - 2 * t**2 + t, -2 * t**3 + 3 * t**2, t**3 - t**2]
+        out[:, j] = h[0] * v1 + h[1] * m1 + h[2] * v2 + h[3] * m2
+    return out
+
+
+def test_resample_straight_strand_is_uniform_line():
+    """Equally spaced collinear points: the control points are equally spaced
+    too, so the curve is the line at uniform speed (a wrong 1/3 or 1/2 factor
+    or a wrong end mirror bends the speed)."""
+    N, M = 9, 41
+    d = np.array([0.3, -0.2, 0.1])
+    P = (np.array([1.0, 2.0, -1.0]) + np.arange(N)[:, None] * d)[None].astype(np.float32)
+    out = oracle.resample32(_strands_from_points(P), N, M)
+    v = np.arange(M) / (M - 1)
+    want = P[0, 0] + v[:, None] * (N - 1) * d
+    np.testing.assert_allclose(out[0, :, :3], want, atol=2e-6)
+    assert np.all(out[0, :, 3] == 1.0)
+
+
+def test_resample_ends_are_the_strand_ends_and_nodes_interpolate():
+    rng = np.random.default_rng(5)
+    N = 12
+    M = 3 * (N - 1) + 1
+    P = np.cumsum(rng.normal(0, 0.1, (4, N, 3)), axis=1).astype(np.float32)
+    out = oracle.resample32(_strands_from_points(P), N, M)
+    assert np.array_equal(out[:, 0, :3], P[:, 0]) and np.array_equal(out[:, -1, :3], P[:, -1])
+    np.testing.assert_allclose(out[:, ::3, :3], P, atol=1e-6)
+
+
+@pytest.mark.parametrize("N,M", [(2, 5), (3, 2), (16, 64), (32, 7)])
+def test_resample_matches_hermite_catmull_rom(N, M):
+    rng = np.random.default_rng(N * 100 + M)
+    P = np.cumsum(rng.normal(0, 0.2, (6, N, 3)), axis=1).astype(np.float32)
+    out = oracle.resample32(_strands_from_points(P), N, M)
+    np.testing.assert_allclose(out[:, :, :3], _catmull_rom64(P, M), atol=3e-6)
