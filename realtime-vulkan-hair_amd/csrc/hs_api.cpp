@@ -749,7 +749,7 @@ hipError_t groom_unpack(const Groom& g, float* aos_dev, hipStream_t st)
     hs::UnpackGrid ug;
     const hs::UnpackGrid* pug = nullptr;
     if (g.pending) {
-        ug.grid = g.gw.grid_i;
+        ug.grid = g.gw.grid_f;
         ug.G = g.G;
         ug.ox = g.origin[0];
         ug.oy = g.origin[1];

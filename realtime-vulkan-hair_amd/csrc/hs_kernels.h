@@ -109,7 +109,7 @@ struct StepArgs {
 };
 
 struct UnpackGrid {
-    const int4* grid;
+    const float4* grid;  // float cells (GridWork::grid_f) of the grid to gather from
     uint32_t G;
     float ox, oy, oz, inv_h, mu, omu;
 };

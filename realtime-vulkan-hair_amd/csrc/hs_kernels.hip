@@ -198,27 +198,6 @@ __device__ __forceinline__ f3 gather_any(const float4* win, const Win& wd, bool 
     return lds ? gather_any_src<true>(win, wd, g, r, q, G) : gather_any_src<false>(win, wd, g, r, q, G);
 }
 
-// From the integer grid, literally as compute.comp:274-293 (read-back path).
-__device__ __forceinline__ f3 gather_global_i(const int4* __restrict__ g, const Rng& r, f3 q, int G)
-{
-    f3 gv = mk(0.0f, 0.0f, 0.0f);
-    for (int a = r.x0; a <= r.x1; ++a) {
-        const float wxa = wgt(q.x, a);
-        for (int b = r.y0; b <= r.y1; ++b) {
-            const float wxy = wxa * wgt(q.y, b);
-            for (int c = r.z0; c <= r.z1; ++c) {
-                const int4 cell = g[(size_t)a + (size_t)b * G + (size_t)c * G * G];
-                if (cell.w > 0) {
-                    const float w = wxy * wgt(q.z, c);
-                    const float t = w * (1.0f / (float)cell.w);
-                    gv = add(gv, smul(t, mk((float)cell.x, (float)cell.y, (float)cell.z)));
-                }
-            }
-        }
-    }
-    return gv;
-}
-
 // ---------------------------------------------------------------------------
 // Splat.  Contributions int(SCALE*(w*v)) and int(SCALE*w) (compute.comp:241-248)
 // accumulate in an LDS window of kSlots copies per voxel (lane & (kSlots-1)
@@ -1297,35 +1276,63 @@ __global__ void k_pack(const float4* __restrict__ aos, const uint32_t* __restric
     }
 }
 
-// SoA -> AoS; when `g` is given, applies the pending grid friction to
-// velocities of points 1..N-1 (compute.comp:257-298) on the way out.
-__global__ void k_unpack(float4* __restrict__ aos, const uint32_t* __restrict__ perm, uint32_t S, uint32_t N,
-                         uint32_t Sp, const float* __restrict__ pos, const float* __restrict__ vel,
-                         const float* __restrict__ corr, const int4* __restrict__ g, uint32_t G, float ox, float oy,
-                         float oz, float inv_h, float mu, float omu)
+// Output tiles: a workgroup stages ts consecutive strand slots (all N points)
+// in LDS, reading each point plane with consecutive lanes on consecutive
+// slots, then writes every strand's block contiguously in caller order.
+// Rows are padded to N+1 entries against LDS bank conflicts.
+constexpr uint32_t kTileF4 = 2048;  // float4 entries of the LDS tile (32 KiB)
+constexpr uint32_t kTileThreads = 256;
+
+// SoA -> AoS (Strand layout); when `g` (the float cells of the last step's
+// grid) is given, applies the pending grid friction to velocities of points
+// 1..N-1 (compute.comp:257-298) on the way.
+__global__ __launch_bounds__(kTileThreads) void k_unpack(float4* __restrict__ aos, const uint32_t* __restrict__ perm,
+                                                         uint32_t S, uint32_t N, uint32_t Sp,
+                                                         const float* __restrict__ pos, const float* __restrict__ vel,
+                                                         const float* __restrict__ corr, const float4* __restrict__ g,
+                                                         uint32_t G, float ox, float oy, float oz, float inv_h,
+                                                         float mu, float omu, uint32_t ts)
 {
-    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= (size_t)S * N) return;
-    const uint32_t p = (uint32_t)(tid / N);
-    const uint32_t i = (uint32_t)(tid % N);
-    const uint32_t s = perm[p];
+    __shared__ float4 tile[kTileF4];
+    const uint32_t p0 = blockIdx.x * ts;
+    const uint32_t ns = min(ts, S - p0);
+    const uint32_t row = N + 1;
     const size_t plane = (size_t)N * Sp;
-    const size_t base = (size_t)s * 3 * N;
-    const size_t o = (size_t)i * Sp + p;
-    const f3 pp = mk(pos[o], pos[plane + o], pos[2 * plane + o]);
-    f3 v = mk(vel[o], vel[plane + o], vel[2 * plane + o]);
-    if (g && i >= 1) {
-        const f3 q = grid_coord(pp, mk(ox, oy, oz), inv_h);
-        const Rng rg = point_range(q, (int)G);
-        f3 gv = mk(0.0f, 0.0f, 0.0f);
-        if (!rng_empty(rg)) gv = gather_global_i(g, rg, q, (int)G);
-        v = friction(v, gv, mu, omu);
+    for (uint32_t arr = 0; arr < 3; ++arr) {
+        for (uint32_t e = threadIdx.x; e < ts * N; e += kTileThreads) {
+            const uint32_t pl = e % ts, i = e / ts;
+            if (pl >= ns) continue;
+            const size_t o = (size_t)i * Sp + p0 + pl;
+            float4 val = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (arr == 0) {
+                val = make_float4(pos[o], pos[plane + o], pos[2 * plane + o], 1.0f);
+            } else if (arr == 1) {
+                f3 v = mk(vel[o], vel[plane + o], vel[2 * plane + o]);
+                if (g && i >= 1) {
+                    const f3 pp = mk(pos[o], pos[plane + o], pos[2 * plane + o]);
+                    const f3 q = grid_coord(pp, mk(ox, oy, oz), inv_h);
+                    const Rng rg = point_range(q, (int)G);
+                    f3 gv = mk(0.0f, 0.0f, 0.0f);
+                    const int Gi = (int)G;
+                    if (rng_full(rg))
+                        gv = gather_block(g, Win{0, 0, 0, Gi, Gi, Gi}, rg, q);  // the whole grid as the window
+                    else if (!rng_empty(rg))
+                        gv = gather_any_src<false>(nullptr, Win{0, 0, 0, 0, 0, 0}, g, rg, q, Gi);
+                    v = friction(v, gv, mu, omu);
+                }
+                val = make_float4(v.x, v.y, v.z, 0.0f);
+            } else if (corr) {
+                val = make_float4(corr[o], corr[plane + o], corr[2 * plane + o], 0.0f);
+            }
+            tile[pl * row + i] = val;
+        }
+        __syncthreads();
+        for (uint32_t e = threadIdx.x; e < ns * N; e += kTileThreads) {
+            const uint32_t pl = e / N, i = e % N;
+            aos[((size_t)perm[p0 + pl] * 3 + arr) * N + i] = tile[pl * row + i];
+        }
+        __syncthreads();
     }
-    aos[base + i] = make_float4(pp.x, pp.y, pp.z, 1.0f);
-    aos[base + N + i] = make_float4(v.x, v.y, v.z, 0.0f);
-    float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (corr) c = make_float4(corr[o], corr[plane + o], corr[2 * plane + o], 0.0f);
-    aos[base + 2 * N + i] = c;
 }
 
 // ---------------------------------------------------------------------------
@@ -1337,47 +1344,64 @@ __global__ void k_unpack(float4* __restrict__ aos, const uint32_t* __restrict__ 
 // v1)); Bezier control points b1 = v1 + (1/3)((v2 - v0)/2), b2 = v2 -
 // (1/3)((v3 - v1)/2); de Casteljau with mix(a,b,t) = a*(1-t) + b*t.  At v = 1
 // the shader would read one point past the strand; here it is the tip
-// (last segment, t = 1).  One thread per (strand slot, sample), the sample
-// index fastest so each strand's output is one contiguous run.
+// (last segment, t = 1).  Positions staged per tile as in k_unpack; samples
+// written strand by strand.
 __device__ __forceinline__ f3 mix3(f3 a, f3 b, float t)
 {
     const float u = 1.0f - t;
     return mk(a.x * u + b.x * t, a.y * u + b.y * t, a.z * u + b.z * t);
 }
 
-__global__ void k_resample(const float* __restrict__ pos, const uint32_t* __restrict__ perm, uint32_t S, uint32_t N,
-                           uint32_t Sp, uint32_t M, float4* __restrict__ out)
+__global__ __launch_bounds__(kTileThreads) void k_resample(const float* __restrict__ pos,
+                                                           const uint32_t* __restrict__ perm, uint32_t S, uint32_t N,
+                                                           uint32_t Sp, uint32_t M, float4* __restrict__ out,
+                                                           uint32_t ts)
 {
-    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= (size_t)S * M) return;
-    const uint32_t p = (uint32_t)(tid / M), j = (uint32_t)(tid % M);
+    __shared__ float4 tile[kTileF4];
+    const uint32_t p0 = blockIdx.x * ts;
+    const uint32_t ns = min(ts, S - p0);
+    const uint32_t row = N + 1;
     const size_t plane = (size_t)N * Sp;
-    auto P = [&](uint32_t i) {
-        const size_t o = (size_t)i * Sp + p;
-        return mk(pos[o], pos[plane + o], pos[2 * plane + o]);
-    };
-    const float v = (float)j / (float)(M - 1);
-    const float x = v * (float)(N - 1);
-    const float fl = floorf(x);
-    uint32_t seg = (uint32_t)fl;
-    float t = x - fl;
-    if (seg >= N - 1) {
-        seg = N - 2;
-        t = 1.0f;
+    for (uint32_t e = threadIdx.x; e < ts * N; e += kTileThreads) {
+        const uint32_t pl = e % ts, i = e / ts;
+        if (pl >= ns) continue;
+        const size_t o = (size_t)i * Sp + p0 + pl;
+        tile[pl * row + i] = make_float4(pos[o], pos[plane + o], pos[2 * plane + o], 0.0f);
     }
-    const f3 v1 = P(seg), v2 = P(seg + 1);
-    const f3 v0 = seg == 0 ? add(v1, sub(v1, v2)) : P(seg - 1);
-    const f3 v3 = seg + 2 == N ? add(v2, sub(v2, v1)) : P(seg + 2);
-    const float third = 1.0f / 3.0f;
-    const f3 b0 = v1;
-    const f3 b1 = add(v1, smul(third, muls(sub(v2, v0), 0.5f)));
-    const f3 b2 = sub(v2, smul(third, muls(sub(v3, v1), 0.5f)));
-    const f3 b3 = v2;
-    const f3 b01 = mix3(b0, b1, t), b11 = mix3(b1, b2, t), b21 = mix3(b2, b3, t);
-    const f3 b02 = mix3(b01, b11, t), b12 = mix3(b11, b21, t);
-    const f3 c = mix3(b02, b12, t);
-    out[(size_t)perm[p] * M + j] = make_float4(c.x, c.y, c.z, 1.0f);
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < ns * M; e += kTileThreads) {
+        const uint32_t pl = e / M, j = e % M;
+        const float4* T = tile + pl * row;
+        auto P = [&](uint32_t i) {
+            const float4 v = T[i];
+            return mk(v.x, v.y, v.z);
+        };
+        const float v = (float)j / (float)(M - 1);
+        const float x = v * (float)(N - 1);
+        const float fl = floorf(x);
+        uint32_t seg = (uint32_t)fl;
+        float t = x - fl;
+        if (seg >= N - 1) {
+            seg = N - 2;
+            t = 1.0f;
+        }
+        const f3 v1 = P(seg), v2 = P(seg + 1);
+        const f3 v0 = seg == 0 ? add(v1, sub(v1, v2)) : P(seg - 1);
+        const f3 v3 = seg + 2 == N ? add(v2, sub(v2, v1)) : P(seg + 2);
+        const float third = 1.0f / 3.0f;
+        const f3 b0 = v1;
+        const f3 b1 = add(v1, smul(third, muls(sub(v2, v0), 0.5f)));
+        const f3 b2 = sub(v2, smul(third, muls(sub(v3, v1), 0.5f)));
+        const f3 b3 = v2;
+        const f3 b01 = mix3(b0, b1, t), b11 = mix3(b1, b2, t), b21 = mix3(b2, b3, t);
+        const f3 b02 = mix3(b01, b11, t), b12 = mix3(b11, b21, t);
+        const f3 c = mix3(b02, b12, t);
+        out[(size_t)perm[p0 + pl] * M + j] = make_float4(c.x, c.y, c.z, 1.0f);
+    }
 }
+
+// Strand slots per output tile: whole padded rows of N + 1 entries.
+static uint32_t tile_strands(uint32_t N) { return max(1u, min(64u, kTileF4 / (N + 1))); }
 
 // The indirect draw record the shader's count produces (compute.comp:302).
 __global__ void k_indirect(uint32_t* __restrict__ rec, uint32_t S)
@@ -1506,10 +1530,10 @@ hipError_t launch_pack(const float* aos_dev, const uint32_t* perm, uint32_t S, u
 hipError_t launch_resample(const float* pos, const uint32_t* perm, uint32_t S, uint32_t N, uint32_t Sp, uint32_t M,
                            float* out_dev, hipStream_t st)
 {
-    if (N < 2 || M < 2) return hipErrorInvalidValue;
-    const size_t n = (size_t)S * M;
-    hipLaunchKernelGGL(k_resample, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, pos, perm, S, N, Sp, M,
-                       reinterpret_cast<float4*>(out_dev));
+    if (N < 2 || M < 2 || N + 1 > kTileF4) return hipErrorInvalidValue;
+    const uint32_t ts = tile_strands(N);
+    hipLaunchKernelGGL(k_resample, dim3((S + ts - 1) / ts), dim3(kTileThreads), 0, st, pos, perm, S, N, Sp, M,
+                       reinterpret_cast<float4*>(out_dev), ts);
     return hipGetLastError();
 }
 
@@ -1522,17 +1546,15 @@ hipError_t launch_indirect(uint32_t* rec_dev, uint32_t S, hipStream_t st)
 hipError_t launch_unpack(float* aos_dev, const uint32_t* perm, uint32_t S, uint32_t N, uint32_t Sp,
                          const float* pos, const float* vel, const float* corr, const UnpackGrid* g, hipStream_t st)
 {
-    const size_t n = (size_t)S * N;
-    const uint32_t bs = 256;
-    const size_t blocks = (n + bs - 1) / bs;
+    if (N + 1 > kTileF4) return hipErrorInvalidValue;
+    const uint32_t ts = tile_strands(N);
+    const dim3 grid((S + ts - 1) / ts), block(kTileThreads);
     if (g)
-        hipLaunchKernelGGL(k_unpack, dim3((uint32_t)blocks), dim3(bs), 0, st, reinterpret_cast<float4*>(aos_dev),
-                           perm, S, N, Sp, pos, vel, corr, g->grid, g->G, g->ox, g->oy, g->oz, g->inv_h, g->mu,
-                           g->omu);
+        hipLaunchKernelGGL(k_unpack, grid, block, 0, st, reinterpret_cast<float4*>(aos_dev), perm, S, N, Sp, pos, vel,
+                           corr, g->grid, g->G, g->ox, g->oy, g->oz, g->inv_h, g->mu, g->omu, ts);
     else
-        hipLaunchKernelGGL(k_unpack, dim3((uint32_t)blocks), dim3(bs), 0, st, reinterpret_cast<float4*>(aos_dev),
-                           perm, S, N, Sp, pos, vel, corr, (const int4*)nullptr, 0u, 0.f, 0.f, 0.f, 0.f, 0.f,
-                           1.f);
+        hipLaunchKernelGGL(k_unpack, grid, block, 0, st, reinterpret_cast<float4*>(aos_dev), perm, S, N, Sp, pos, vel,
+                           corr, (const float4*)nullptr, 0u, 0.f, 0.f, 0.f, 0.f, 0.f, 1.f, ts);
     return hipGetLastError();
 }
 
