@@ -332,6 +332,33 @@ void hs_collider_from_trs(hs_collider* out, const float translation[3],
 void hs_collider_translate(hs_collider* c, const float translation[3]);
 
 /* ------------------------------------------------------------------------ */
+/* Groom ingestion (host only; Strand.cpp:10-191, tinyobjloader 2.0)         */
+/* ------------------------------------------------------------------------ */
+
+/* The triangles of an OBJ file in file order, polygons triangulated the way
+ * tinyobjloader 2.0 does with triangulate = true (ear clipping in the
+ * dominant plane; a convex quad gives corners 0,1,2 then 0,2,3), numbers read
+ * as it reads them: 9 floats of corner positions and 9 of corner normals per
+ * triangle (GeneratePointsOnMesh's `positions` / `normals`, Strand.cpp:55-77).
+ * Sets *num_triangles; fills the arrays when both are non-NULL and
+ * `capacity` (triangles) suffices.  HS_ERR_NOT_FOUND: no such file;
+ * HS_ERR_INVALID: a bad index or a corner without a normal. */
+int hs_obj_triangles(const char* path, float* positions, float* normals, uint32_t capacity,
+                     uint32_t* num_triangles);
+/* Hair roots on those triangles (Strand.cpp:88-107): srand(seed) with MSVC's
+ * rand() sequence (x = x*214013 + 2531011, bits 16..30; the reference seeds
+ * 8), per strand a triangle rand() % T and barycentrics rand()/RAND_MAX folded
+ * into it; the root normal is the triangle's first corner normal.  3 floats
+ * per strand in each output. */
+int hs_sample_roots(const float* positions, const float* normals, uint32_t num_triangles, uint32_t num_strands,
+                    uint32_t seed, float* roots, float* root_normals);
+/* Initial strands of Hair::Hair (Strand.cpp:157-172) in the Strand layout:
+ * point j = root + j * (float)(length / (N - 1.0)) * (normal + (0.05, 5, -2)),
+ * velocities (0, 0, -1), corrections 0.  48*N bytes per strand. */
+int hs_initial_strands(const float* roots, const float* root_normals, uint32_t num_strands, uint32_t num_points,
+                       float length, float* strands_aos);
+
+/* ------------------------------------------------------------------------ */
 /* Served simulation jobs (no reference counterpart: the reference steps one  */
 /* groom per frame, main.cpp:257-285).  A server owns a context, an HBM arena */
 /* of `hbm_budget` bytes and three threads: admission (uploads jobs in FIFO   */

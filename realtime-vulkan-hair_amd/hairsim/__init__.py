@@ -186,6 +186,12 @@ _SIGS = {
     "hs_profile_breakdown": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "hs_profile_mesh": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "hs_export_strands": (C.c_int, [_P, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "hs_obj_triangles": (C.c_int, [C.c_char_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint32,
+                                   C.POINTER(C.c_uint32)]),
+    "hs_sample_roots": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint32, C.c_uint32, C.c_uint32,
+                                  C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "hs_initial_strands": (C.c_int, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint32, C.c_uint32, C.c_float,
+                                     C.POINTER(C.c_float)]),
     "hs_resample_strands": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_void_p]),
     "hs_collider_from_trs": (None, [C.POINTER(HsCollider), C.POINTER(C.c_float), C.POINTER(C.c_float),
                                     C.POINTER(C.c_float)]),
@@ -467,6 +473,46 @@ class Batch:
 
     def step(self):
         self.ctx._check(lib().hs_step_batch(self.ctx._h, self._items, self.n))
+
+
+def obj_triangles(path):
+    """(T, 3, 3) corner positions and (T, 3, 3) corner normals of an OBJ file
+    (triangulated as tinyobjloader 2.0 does)."""
+    n = C.c_uint32()
+    rc = lib().hs_obj_triangles(os.fsencode(path), None, None, 0, C.byref(n))
+    if rc != HS_OK:
+        raise HairSimError(rc, f"hs_obj_triangles({path!r})")
+    pos = np.empty((n.value, 3, 3), dtype=np.float32)
+    nrm = np.empty((n.value, 3, 3), dtype=np.float32)
+    rc = lib().hs_obj_triangles(os.fsencode(path), _fptr(pos), _fptr(nrm), n.value, C.byref(n))
+    if rc != HS_OK:
+        raise HairSimError(rc, f"hs_obj_triangles({path!r})")
+    return pos, nrm
+
+
+def sample_roots(positions, normals, num_strands, seed=8):
+    """Roots and root normals (S, 3) as GeneratePointsOnMesh samples them."""
+    p = np.ascontiguousarray(positions, dtype=np.float32)
+    n = np.ascontiguousarray(normals, dtype=np.float32)
+    T = p.size // 9
+    roots = np.empty((num_strands, 3), dtype=np.float32)
+    rn = np.empty((num_strands, 3), dtype=np.float32)
+    rc = lib().hs_sample_roots(_fptr(p), _fptr(n), T, num_strands, seed, _fptr(roots), _fptr(rn))
+    if rc != HS_OK:
+        raise HairSimError(rc, "hs_sample_roots")
+    return roots, rn
+
+
+def initial_strands(roots, root_normals, num_points, length=2.5):
+    """Strand-layout buffer of Hair::Hair's initial strands."""
+    r = np.ascontiguousarray(roots, dtype=np.float32)
+    n = np.ascontiguousarray(root_normals, dtype=np.float32)
+    S = r.size // 3
+    out = np.empty(S * 12 * num_points, dtype=np.float32)
+    rc = lib().hs_initial_strands(_fptr(r), _fptr(n), S, num_points, length, _fptr(out))
+    if rc != HS_OK:
+        raise HairSimError(rc, "hs_initial_strands")
+    return out
 
 
 _hip = None

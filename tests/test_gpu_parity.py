@@ -223,3 +223,21 @@ def test_stacked_voxel_near_limit_and_list_overflow(ctx, S):
             assert gpu.grid_near_limit() == (k + 1) * near
     finally:
         gpu.destroy()
+
+
+def test_reference_demo_groom_bitwise(ctx):
+    """The reference demo end to end: its 900 roots on the mannequin mesh
+    (tests/golden/mannequin_roots.npz, srand(8) over the triangulated OBJ),
+    Hair::Hair's 10-point strands (hs_initial_strands), the six demo colliders
+    and the reference constants; 30 steps bitwise vs ref32."""
+    import os
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "mannequin_roots.npz"))
+    N = 10
+    S = fx["roots"].shape[0]
+    aos = hairsim.initial_strands(fx["roots"], fx["normals"], N).reshape(S, 12 * N)
+    g = scenes.Groom(aos, N, np.full(S, np.float32(2.5 / 9.0)), None, grid_dim=64, grid_origin=(-3.0, -2.0, -5.0),
+                     grid_h=7.0 / 64.0)
+    wl = scenes.Workload("reference-demo-mannequin", dt=1.0 / 60.0, steps=30, damping=0.998, friction=0.08,
+                         hair_hair=True, sphere_radius=1.0, grooms=[g])
+    cols = reference_demo_colliders()
+    run_parity(ctx, wl, 30, check_every=5, ncol=6, colliders_fn=lambda k: cols)
