@@ -255,7 +255,7 @@ int hs_export_strands(hs_ctx* ctx, uint32_t groom_id, float* strands_dev, uint32
  * (where the shader would read one point past the strand).  Positions as
  * float4 (w = 1), S*M*16 bytes of DEVICE memory `out_dev`, caller strand
  * order.  Enqueued on the library stream. */
-int hs_resample_strands(hs_ctx* ctx, uint32_t groom_id, uint32_t samples, float* out_dev);
+int hs_resample_strands(hs_ctx* ctx, uint32_t groom_id, uint32_t samples, float* out_dev);  /* 2 <= samples <= 2^20 */
 /* Overwrite the groom's state from a reference-layout buffer (checkpoint
  * restore; clears any pending grid friction). */
 int hs_write_strands(hs_ctx* ctx, uint32_t groom_id, const float* strands_aos);

@@ -1255,10 +1255,9 @@ int hs_resample_strands(hs_ctx* c, uint32_t id, uint32_t M, float* out_dev)
 {
     if (!c) return HS_ERR_INVALID;
     if (!out_dev) return fail(c, HS_ERR_INVALID, "hs_resample_strands: null buffer");
-    if (M < 2) return fail(c, HS_ERR_INVALID, "hs_resample_strands: samples must be >= 2");
+    if (M < 2 || M > (1u << 20)) return fail(c, HS_ERR_INVALID, "hs_resample_strands: samples must be in [2, 2^20]");
     Groom* g = find(c, id);
     if (!g) return fail(c, HS_ERR_NOT_FOUND, "hs_resample_strands: unknown groom");
-    if ((uint64_t)g->S * M >= (1ull << 40)) return fail(c, HS_ERR_INVALID, "hs_resample_strands: too many samples");
     HS_TRY(c, hipSetDevice(c->device));
     HS_TRY(c, hs::launch_resample(g->pos, g->perm, g->S, g->N, g->L.Sp, M, out_dev, c->stream));
     return HS_OK;
