@@ -224,17 +224,24 @@ int hs_obj_triangles(const char* path, float* positions, float* normals, uint32_
     if (!path || !num_triangles) return HS_ERR_INVALID;
     FILE* f = std::fopen(path, "rb");
     if (!f) return HS_ERR_NOT_FOUND;
+    std::string text;
+    {
+        char buf[1 << 16];
+        size_t got;
+        while ((got = std::fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, got);
+    }
+    std::fclose(f);
     std::vector<float> V, VN;
     std::vector<Corner> tris;
-    std::vector<char> line(1 << 16);
     bool bad = false;
-    std::string acc;
-    while (std::fgets(line.data(), (int)line.size(), f)) {
-        acc += line.data();
-        if (acc.empty() || (acc.back() != '\n' && !std::feof(f))) continue;  // a line longer than the buffer
-        const char* p = acc.c_str();
-        const char* end = p + acc.size();
-        while (end > p && (end[-1] == '\n' || end[-1] == '\r')) --end;
+    size_t at = 0;
+    while (at < text.size() && !bad) {
+        size_t nl = text.find('\n', at);
+        if (nl == std::string::npos) nl = text.size();
+        const char* p = text.data() + at;
+        const char* end = text.data() + nl;
+        at = nl + 1;
+        while (end > p && end[-1] == '\r') --end;
         while (p < end && (*p == ' ' || *p == '\t')) ++p;
         if (end - p >= 2 && p[0] == 'v' && (p[1] == ' ' || p[1] == '\t')) {
             p += 2;
@@ -270,9 +277,7 @@ int hs_obj_triangles(const char* path, float* positions, float* normals, uint32_
             if (bad) break;
             triangulate(poly, V, &tris);
         }
-        acc.clear();
     }
-    std::fclose(f);
     if (bad) return HS_ERR_INVALID;
     const size_t nt = tris.size() / 3;
     for (const Corner& c : tris)

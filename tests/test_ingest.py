@@ -162,3 +162,10 @@ def test_reference_mannequin_demo_groom():
     assert np.array_equal(roots.view(np.uint32), r2.view(np.uint32)) and np.array_equal(rn, n2)
     fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "mannequin_roots.npz"))
     assert np.array_equal(fx["roots"], roots) and np.array_equal(fx["normals"], rn)
+
+
+def test_last_line_without_newline_and_crlf(tmp_path):
+    """The final face line may lack a newline; CRLF line ends are accepted."""
+    text = "v 0 0 0\r\nv 1 0 0\r\nv 0 1 0\r\nvn 0 0 1\r\nf 1//1 2//1 3//1"
+    p, n = hairsim.obj_triangles(_write(tmp_path, text))
+    assert p.shape == (1, 3, 3) and np.array_equal(n[0, 0], np.array([0, 0, 1], dtype=np.float32))
