@@ -76,7 +76,7 @@ bool parse_number(const char* s, const char* e, double* out)
         }
         int n = 0;
         while (p != e && is_digit(*p)) {
-            ex = ex * 10 + (*p - '0');
+            if (ex < 100000) ex = ex * 10 + (*p - '0');  // beyond |e| = 1e5 the value is 0 or inf either way
             ++p;
             ++n;
         }
