@@ -297,7 +297,8 @@ typedef struct hs_mesh_desc {
 
 /* vertices: num_vertices x 3 floats; triangles: num_triangles x 3 indices,
  * counter-clockwise seen from outside; the mesh must be closed (every edge
- * shared by exactly two triangles).  Copies everything. */
+ * shared by exactly two triangles) with finite coordinates.  Copies
+ * everything. */
 int hs_mesh_create(hs_ctx* ctx, const float* vertices, uint32_t num_vertices, const uint32_t* triangles,
                    uint32_t num_triangles, const hs_mesh_desc* desc, uint32_t* mesh_id);
 int hs_mesh_destroy(hs_ctx* ctx, uint32_t mesh_id);

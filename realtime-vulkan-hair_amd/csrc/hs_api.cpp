@@ -794,7 +794,8 @@ int hs_mesh_create(hs_ctx* c, const float* v, uint32_t nv, const uint32_t* t, ui
             return fail(c, HS_ERR_INVALID, "hs_mesh_create: grid_dim must be in [1, 1024]");
     hsmesh::Built b;
     if (!hsmesh::build(v, nv, t, nt, d->grid_dim, d->margin, &b))
-        return fail(c, HS_ERR_INVALID, "hs_mesh_create: mesh is not closed (or an index is out of range)");
+        return fail(c, HS_ERR_INVALID,
+                    "hs_mesh_create: mesh is not closed (or an index is out of range, or a vertex is not finite)");
     HS_TRY(c, hipSetDevice(c->device));
     const uint64_t tri_bytes = round_up(b.tris.size() * 4, 256);
     const uint64_t sph_bytes = round_up(b.spheres.size() * 4, 256);

@@ -51,13 +51,16 @@ inline D3 unit(D3 a)
 }
 
 // Returns false for a mesh that is not a closed 2-manifold (an edge not
-// shared by exactly two triangles) or has an index out of range.
+// shared by exactly two triangles), has an index out of range or a
+// non-finite vertex coordinate.
 inline bool build(const float* v, uint32_t nv, const uint32_t* t, uint32_t nt, const uint32_t dims[3], float margin,
                   Built* out)
 {
     if (nv < 4 || nt < 4) return false;
     for (uint64_t k = 0; k < 3ull * nt; ++k)
         if (t[k] >= nv) return false;
+    for (uint64_t k = 0; k < 3ull * nv; ++k)
+        if (!std::isfinite(v[k])) return false;
     auto V = [&](uint32_t i) { return D3{v[3 * i], v[3 * i + 1], v[3 * i + 2]}; };
     std::vector<D3> fn(nt), vn(nv, D3{0, 0, 0});
     std::map<std::pair<uint32_t, uint32_t>, std::vector<uint32_t>> edges;

@@ -96,6 +96,11 @@ def test_mesh_errors(ctx):
     with pytest.raises(hairsim.HairSimError) as e:
         ctx.mesh(V, T[:-1])
     assert e.value.code == hairsim.HS_ERR_INVALID
+    Vn = V.copy()
+    Vn[3, 1] = np.nan
+    with pytest.raises(hairsim.HairSimError) as e:
+        ctx.mesh(Vn, T)
+    assert e.value.code == hairsim.HS_ERR_INVALID
     wl = scenes.plain_1k(S=64)
     g = wl.grooms[0]
     gpu = ctx.groom(g.strands, g.N)
