@@ -1076,8 +1076,8 @@ __global__ __launch_bounds__(kW) void k_step_batch(const StepArgs* __restrict__ 
     const uint32_t gwg = swz ? xcd_swizzle(blockIdx.x, total) : blockIdx.x;
     if (gwg >= total) return;
     const uint32_t g = find_groom(wg_start, ngroom, gwg);
-    // by-value copy: the arguments then live in SGPRs like a kernel argument
-    const StepArgs a = args[g];
+    // read in place (scalar loads from the argument blob where used)
+    const StepArgs& a = args[g];
     step_body<GATHER, SPLAT, COL>(a, gwg - wg_start[g]);
 }
 
