@@ -1238,9 +1238,11 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce_batch(const StepArgs*
                                                                  uint32_t ngroom)
 {
     const uint32_t total = brick_start[ngroom];
-    const uint32_t gb = blockIdx.x * kBricksPerWG + (threadIdx.x >> 6);
+    // one brick per wavefront: the index is wave-uniform, and saying so keeps
+    // the groom lookup and its arguments in scalar registers
+    const uint32_t gb = __builtin_amdgcn_readfirstlane(blockIdx.x * kBricksPerWG + (threadIdx.x >> 6));
     if (gb >= total) return;
-    const uint32_t g = find_groom(brick_start, ngroom, gb);
+    const uint32_t g = __builtin_amdgcn_readfirstlane(find_groom(brick_start, ngroom, gb));
     const uint32_t b = gb - brick_start[g];
     reduce_brick(args[g].gw, args[g].G, args[g].parity, b, b == 0);
 }
