@@ -42,6 +42,26 @@ def test_export_equals_read_back(ctx):
         gpu.destroy()
 
 
+def test_export_with_kept_corrections(ctx):
+    """A groom that keeps its correction vectors exports them too."""
+    wl = scenes.plain_100k(S=300, N=12, G=16)
+    g = wl.grooms[0]
+    gpu = make_gpu_groom(ctx, wl, keep_corrections=True)
+    for k in range(4):
+        gpu.step(wl.dt, float(wl.time(k)), [scenes.sphere_collider(wl.sphere(k))], gpu_params(wl, k))
+    buf = hairsim.DeviceBuffer(48 * g.S * g.N)
+    try:
+        gpu.export(buf)
+        ctx.sync()
+        host, _ = gpu.read()
+        out = buf.numpy(np.float32)
+        assert_bitwise(out, host, "export with corrections", g.N)
+        assert np.any(out.reshape(-1, 3, g.N, 4)[:, 2] != 0)
+    finally:
+        buf.free()
+        gpu.destroy()
+
+
 @pytest.mark.parametrize("S,N,M", [(1000, 16, 64), (130, 2, 5), (257, 32, 7), (64, 10, 2)])
 def test_resample_bitwise(ctx, S, N, M):
     wl = scenes.plain_100k(S=S, N=N, G=16)

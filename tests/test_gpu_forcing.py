@@ -80,3 +80,31 @@ def test_forcing_mode_out_of_range_is_rejected(ctx):
         assert e.value.code == hairsim.HS_ERR_INVALID
     finally:
         gpu.destroy()
+
+
+def test_forcing_with_mesh_and_ellipsoid_bitwise(ctx):
+    """Forcing (mode 2), the mesh pass and an ellipsoid in one step: the
+    force terms add in the shader's order (gravity + wind + noise, forcing,
+    then the collider average with the mesh last)."""
+    wl = scenes.mesh_head_scene("forcing-mesh", 41, 700, 12, 16, subdiv=3, margin=0.06, mesh_grid=(12, 14, 12),
+                                steps=12)
+    g = wl.grooms[0]
+    V, T = wl.mesh
+    mid = ctx.mesh(V, T, grid_dim=wl.mesh_grid, margin=wl.mesh_margin)
+    recs = oracle.mesh_prepare(V, T)
+    ell = hairsim.collider_from_trs((0.4, 0.9, 0.3), (0.0, 30.0, 10.0), (0.5, 0.4, 0.6)).as_array()
+    gpu = make_gpu_groom(ctx, wl)
+    ref = g.strands.copy()
+    grid = None
+    try:
+        for k in range(12):
+            t = float(np.float32(wl.time(k) + 2.1 * k))
+            cols = [scenes.sphere_collider(wl.sphere(k)), ell]
+            gpu.step(wl.dt, t, cols, gpu_params(wl, k, ncol=2, forcing=2, mesh_id=mid))
+            grid = oracle.step32(oracle_params(wl, k, colliders=np.stack(cols), total_time=t, forcing=2, mesh=recs,
+                                               mesh_margin=wl.mesh_margin), ref, g.N, grid)
+        out, _ = gpu.read()
+        assert_bitwise(_strip_corr(out, g.N), _strip_corr(ref, g.N), "forcing + mesh + ellipsoid", g.N)
+    finally:
+        gpu.destroy()
+        ctx.mesh_destroy(mid)
