@@ -349,8 +349,9 @@ int hs_obj_triangles(const char* path, float* positions, float* normals, uint32_
 /* Hair roots on those triangles (Strand.cpp:88-107): srand(seed) with MSVC's
  * rand() sequence (x = x*214013 + 2531011, bits 16..30; the reference seeds
  * 8), per strand a triangle rand() % T and barycentrics rand()/RAND_MAX folded
- * into it; the root normal is the triangle's first corner normal.  3 floats
- * per strand in each output. */
+ * into it; the root normal is the triangle's first corner normal.  As in the
+ * reference, rand() <= 32767 means only the first 32768 triangles can get
+ * roots.  3 floats per strand in each output; 1 <= T <= 2^31 - 1. */
 int hs_sample_roots(const float* positions, const float* normals, uint32_t num_triangles, uint32_t num_strands,
                     uint32_t seed, float* roots, float* root_normals);
 /* Initial strands of Hair::Hair (Strand.cpp:157-172) in the Strand layout:

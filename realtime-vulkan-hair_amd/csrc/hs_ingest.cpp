@@ -297,7 +297,8 @@ int hs_obj_triangles(const char* path, float* positions, float* normals, uint32_
 int hs_sample_roots(const float* positions, const float* normals, uint32_t num_triangles, uint32_t num_strands,
                     uint32_t seed, float* roots, float* root_normals)
 {
-    if (!positions || !normals || !roots || !root_normals || num_triangles == 0) return HS_ERR_INVALID;
+    if (!positions || !normals || !roots || !root_normals || num_triangles == 0 || num_triangles > INT32_MAX)
+        return HS_ERR_INVALID;
     MsvcRand r{seed};  // srand(seed) (Strand.cpp:88 uses 8)
     for (uint32_t i = 0; i < num_strands; ++i) {
         // Strand.cpp:89-107: triangle rand() % T, barycentrics rand()/RAND_MAX folded into the triangle

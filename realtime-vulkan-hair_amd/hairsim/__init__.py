@@ -560,6 +560,13 @@ class DeviceBuffer:
             _hip_runtime().hipFree(self.ptr)
             self.ptr = None
 
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.free()
+        return False
+
 
 def _dev(buf):
     if buf is None:
