@@ -36,22 +36,23 @@ def _job_set(seed, n):
     return jobs
 
 
-def _params(ncol=1):
-    return hairsim.step_params(damping=0.9, sphere_radius=0.3, friction=0.05, num_colliders=ncol, hair_hair=True)
+def _params(ncol=1, forcing=0):
+    return hairsim.step_params(damping=0.9, sphere_radius=0.3, friction=0.05, num_colliders=ncol, hair_hair=True,
+                               forcing=forcing)
 
 
 GRID = dict(grid_dim=16, grid_origin=(-2.2, -2.2, -2.2), grid_h=4.4 / 16)
 DT = 1.0 / 60.0
 
 
-def _alone(ctx, job):
+def _alone(ctx, job, forcing=0):
     gr = ctx.groom(job["aos"], job["N"], seg_length=job["seg"], wind_noise=job["noise"], **GRID)
     karr = hairsim.keyframes(job["keys"])
     for k in range(job["steps"]):
         t = float(np.float32(k + 1) * np.float32(DT))
         wind, sph = hairsim.keyframes_eval(karr, t)
         p = hairsim.step_params(wind=wind, damping=0.9, sphere_radius=0.3, friction=0.05, num_colliders=1,
-                                hair_hair=True)
+                                hair_hair=True, forcing=forcing)
         gr.step(DT, t, [scenes.sphere_collider(sph)], p)
     out, _ = gr.read()
     gr.destroy()
@@ -60,14 +61,15 @@ def _alone(ctx, job):
 
 def test_served_jobs_equal_alone(ctx):
     jobs = _job_set(77, 10)
-    refs = [_alone(ctx, j) for j in jobs]
+    # a third of the jobs each with no forcing, the sinusoidal and the fbm wind
+    refs = [_alone(ctx, j, forcing=i % 3) for i, j in enumerate(jobs)]
     sizes = [hairsim.groom_bytes(j["aos"].size // (12 * j["N"]), j["N"], 16) + j["aos"].nbytes for j in jobs]
     budget = int(sum(sorted(sizes)[-3:]) * 1.05)  # about three jobs resident at once
     with hairsim.Server(0, hbm_budget=budget, max_points_per_step=60000) as srv:
         ids = []
         for i, j in enumerate(jobs):
-            jid, _ = srv.submit(j["aos"], j["N"], j["steps"], DT, j["keys"], _params(), seg_length=j["seg"],
-                                wind_noise=j["noise"], **GRID)
+            jid, _ = srv.submit(j["aos"], j["N"], j["steps"], DT, j["keys"], _params(forcing=i % 3),
+                                seg_length=j["seg"], wind_noise=j["noise"], **GRID)
             ids.append(jid)
             if i % 3 == 2:
                 time.sleep(0.01)  # later jobs arrive while earlier ones run
