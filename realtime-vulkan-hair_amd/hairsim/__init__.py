@@ -548,7 +548,8 @@ class DeviceBuffer:
         _hip_runtime().hipMemset(self.ptr, int(byte), self.nbytes)
 
     def numpy(self, dtype):
-        """Blocking copy to the host."""
+        """Blocking copy to the host.  It does not wait for the library's
+        stream: call Context.sync() after the calls that write the buffer."""
         out = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype=dtype)
         rc = _hip_runtime().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2)  # device to host
         if rc != 0:
