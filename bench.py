@@ -44,15 +44,18 @@ def make_params(hairsim, wl, k, mesh_id=0):
                                friction=wl.friction, num_colliders=1, hair_hair=wl.hair_hair, mesh_id=mesh_id)
 
 
-def cpu_baseline(wl, budget_s=12.0):
-    """Time the FP32 oracle (OpenMP over strands) on a bounded sample of the workload."""
+def cpu_baseline(wl, budget_s=15.0):
+    """Time the FP32 oracle (OpenMP over strands) on a bounded sample of the
+    workload: every strand (a strand subset with a mesh collider, which the
+    oracle checks by scanning every triangle), as many steps as fit in
+    budget_s."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # the checker, used here only as the timed CPU baseline
     from hairsim import scenes
 
     g = wl.grooms[0]
     # the oracle scans every mesh triangle per point: a smaller sample then
-    S = min(g.S, 2000 if wl.mesh is not None else 20000)
+    S = min(g.S, 2000 if wl.mesh is not None else 1_000_000)
     strands = g.strands[:S].copy()
     seg = g.seg[:S].copy()
     noise = None if g.noise is None else g.noise[:S].copy()
