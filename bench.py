@@ -50,7 +50,7 @@ def cpu_baseline(wl, budget_s=15.0):
     oracle checks by scanning every triangle), as many steps as fit in
     budget_s."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle  # the checker, used here only as the timed CPU baseline
+    import oracle  # the oracle, used here only as the timed CPU baseline
     from hairsim import scenes
 
     g = wl.grooms[0]
@@ -504,7 +504,7 @@ def main():
                     help="run only a plain elementwise kernel over MIB MiB (byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+        print(f"bench.py: this benchmark runs on one GPU only (asked for --gpus {args.gpus}); "
               "the strand step has no multi-GPU path here", file=sys.stderr)
         sys.exit(2)
     if args.copy_probe:

@@ -1,7 +1,7 @@
 /*
  * hair_oracle.h - CPU oracle for the strand-physics pass (TEST INFRASTRUCTURE).
  *
- * This is the checker, never the product: only tests/, the smoke check and
+ * This is the parity reference, never the product: only tests/, the smoke check and
  * bench.py's CPU-baseline leg may load it.  It restates the reference's
  * compute shader src/shaders/compute.comp:124-303 line by line on the
  * reference's own AoS `Strand` layout (compute.comp:44-48), once in binary32
@@ -72,7 +72,7 @@ typedef struct ho_params {
 } ho_params;
 
 /* Mesh collider.  The reference has no mesh collider (compute.comp:155-184
- * handles a sphere and ellipsoids); the rule restated here is this build's:
+ * handles a sphere and ellipsoids); the rule restated here is the library's:
  *   q = the point of the mesh nearest to x, over every triangle (closest point
  *       on a triangle: Ericson, Real-Time Collision Detection, 2005, 5.1.5),
  *       the lowest triangle index winning ties;

@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/libhair_oracle.so).
 
 TEST INFRASTRUCTURE: only tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg import this module, and only as the checker.  It restates
+cpu_baseline leg import this module, and only as the parity reference.  It restates
 src/shaders/compute.comp:124-303 (see hair_oracle.h for the arithmetic
 contract); ``step32`` is "ref32", ``step64`` is "oracle64".
 """
