@@ -495,7 +495,7 @@ def main():
     ap.add_argument("--workload", default="plain-100k",
                     choices=["plain-100k", "large-1m", "plain-1k", "batched-256", "served", "export-100k"])
     ap.add_argument("--rate", type=float, default=2000.0, help="served: job arrivals per second")
-    ap.add_argument("--seconds", type=float, default=10.0, help="served: arrival window")
+    ap.add_argument("--seconds", type=float, default=30.0, help="served: arrival window")
     ap.add_argument("--max-points", type=int, default=12_000_000, help="served: points per batched launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="batched: eager launches instead of graphs")
