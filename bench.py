@@ -9,15 +9,22 @@ dt 1/60.  A "step" is one full physics step of the groom (predict, FTL, DFTL,
 collider penalty, grid splat, grid friction).  State is resident in HBM before
 the timed region starts.
 
+The scene is 240 steps (step k sees totalTime (k+1) dt); --warmup W (default
+10) steps run untimed, then --steps K (default: the rest of the scene) are
+timed, so by default the timed steps are the scene's steps 10..239.
+
 Prints ONE JSON line.  `value` is point-updates/s (S*N per step) over the
 timed steps (wall clock between two stream synchronisations).  `roofline`
 prices the dominant kernel, the strand step kernel, by its algorithmic bytes
 (48 B per non-root point + 16 B per strand) over its average duration,
 measured with HIP events the library records on the stream the kernel runs
-on; the voxel reduce kernel's share (32 B per voxel) and the whole step's
-figure are given beside it.  `cpu_baseline` times the oracle's FP32
-restatement of the reference shader on the host cores on a bounded sample of
-the same workload.
+on, over the same scene steps replayed from a fresh copy of the groom (so
+the kernel averages describe exactly the timed steps; the replay must end
+bit-identical to the timed run); the voxel reduce kernel's share (32 B per
+voxel) and the whole step's figure are given beside it.  `traffic` is read
+from a committed rocprofv3 counter summary, named in `traffic_source`.
+`cpu_baseline` times the oracle's FP32 restatement of the reference shader on
+the host cores on a bounded sample of the same workload.
 
 --workload batched-256: 256 grooms of uneven shapes stepped together with one
 captured graph launch per step, reported beside the same grooms stepped with
@@ -97,58 +104,90 @@ def copy_probe(mib, reps=5):
 
 
 def load_traffic(name):
+    """HBM bytes per dispatch of a kernel and where the figure comes from.
+
+    Counters cannot be read from inside this process: the figure is taken from
+    a committed summary of two rocprofv3 --pmc runs (FETCH_SIZE, then
+    WRITE_SIZE) of the command named there, scaled by the calibration factors
+    it lists.  (None, None) when no summary exists."""
     prof = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
     if not os.path.exists(prof):
-        return None
+        return None, None
     with open(prof) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+        t = json.load(f)
+    src = (f"profiles/traffic_{name}.json (committed rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs"
+           f"{' of: ' + t['command'] if t.get('command') else ''}"
+           f"{', dispatches ' + t['dispatches'] if t.get('dispatches') else ''}; not measured in this run)")
+    return t.get("hbm_bytes_per_launch"), src
+
+
+def scene_steps(args, wl):
+    """(warmup, steps) for a single-groom workload: by default the whole
+    scene, its first 10 steps untimed."""
+    w = 10 if args.warmup is None else args.warmup
+    k = max(1, wl.steps - w) if args.steps is None else args.steps
+    return w, k
+
+
+def run_scene(ctx, hairsim, scenes, wl, mesh_id, warmup, steps, profile):
+    """Steps 0..warmup-1 untimed, then steps warmup..warmup+steps-1 between two
+    stream synchronisations, on a fresh copy of the groom.  Scene step k sees
+    totalTime (k+1)*dt and the wind and sphere of that time.  With `profile`
+    the library brackets each kernel of the timed steps with HIP events.
+    Returns (wall seconds of the timed steps, kernel ms dict, final state,
+    near-limit count)."""
+    g = wl.grooms[0]
+    groom = ctx.groom(g.strands, g.N, seg_length=g.seg, wind_noise=g.noise, noise_dir=g.noise_dir,
+                      grid_dim=g.grid_dim, grid_origin=g.grid_origin, grid_h=g.grid_h)
+    total = warmup + steps
+    # per-step inputs prepared ahead so the timed loop is launches only
+    cols = [[hairsim.HsCollider.from_buffer_copy(scenes.sphere_collider(wl.sphere(k)).tobytes())]
+            for k in range(total)]
+    prm = [make_params(hairsim, wl, k, mesh_id) for k in range(total)]
+    tms = [float(wl.time(k)) for k in range(total)]
+    for k in range(warmup):
+        groom.step(wl.dt, tms[k], cols[k], prm[k])
+    ctx.sync()
+    if profile:
+        ctx.profile_enable(True)
+    t0 = time.perf_counter()
+    for k in range(warmup, total):
+        groom.step(wl.dt, tms[k], cols[k], prm[k])
+    ctx.sync()
+    el = time.perf_counter() - t0
+    kern = None
+    if profile:
+        n_launch, _ = ctx.profile_read()
+        step_ms, reduce_ms = ctx.profile_breakdown()
+        kern = {"launches": n_launch, "step_ms": step_ms, "reduce_ms": reduce_ms, "mesh_ms": ctx.profile_mesh()}
+        ctx.profile_enable(False)
+    out, _ = groom.read()
+    near = groom.grid_near_limit() if g.grid_dim and wl.hair_hair else 0
+    groom.destroy()
+    return el, kern, out, near
 
 
 def bench_single(args, hairsim, scenes, wl):
     g = wl.grooms[0]
+    warmup, steps = scene_steps(args, wl)
+    args.warmup, args.steps = warmup, steps
     ctx = hairsim.Context(0)
-    groom = ctx.groom(g.strands, g.N, seg_length=g.seg, wind_noise=g.noise, noise_dir=g.noise_dir,
-                      grid_dim=g.grid_dim, grid_origin=g.grid_origin, grid_h=g.grid_h)
     mesh_id = 0
     if wl.mesh is not None:
         mesh_id = ctx.mesh(*wl.mesh, grid_dim=wl.mesh_grid, margin=wl.mesh_margin)
-    # per-step inputs prepared ahead so the timed loop is launches only
-    total = args.warmup + 2 * args.steps
-    cols = [[hairsim.HsCollider.from_buffer_copy(scenes.sphere_collider(wl.sphere(k % wl.steps)).tobytes())]
-            for k in range(total)]
-    prm = [make_params(hairsim, wl, k % wl.steps, mesh_id) for k in range(total)]
-    tms = [float(wl.time(k % wl.steps)) for k in range(total)]
-
-    k = 0
-    for _ in range(args.warmup):
-        groom.step(wl.dt, tms[k], cols[k], prm[k])
-        k += 1
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        groom.step(wl.dt, tms[k], cols[k], prm[k])
-        k += 1
-    ctx.sync()
-    el = time.perf_counter() - t0
-
-    # kernel durations: the same number of steps again with HIP events on the
-    # library's stream around each kernel
-    ctx.profile_enable(True)
-    for _ in range(args.steps):
-        groom.step(wl.dt, tms[k], cols[k], prm[k])
-        k += 1
-    ctx.sync()
-    n_launch, _ = ctx.profile_read()
-    step_ms, reduce_ms = ctx.profile_breakdown()
-    mesh_ms = ctx.profile_mesh()
-    ctx.profile_enable(False)
-    out, _ = groom.read()
-    near = groom.grid_near_limit() if g.grid_dim and wl.hair_hair else 0
+    # the timed run, then the same scene steps again from a fresh copy of the
+    # groom with kernel events: the kernel averages describe the timed steps
+    el, _, out, near = run_scene(ctx, hairsim, scenes, wl, mesh_id, warmup, steps, profile=False)
+    _, kern, out2, _ = run_scene(ctx, hairsim, scenes, wl, mesh_id, warmup, steps, profile=True)
+    ctx.close()
     if not np.isfinite(out).all():
         print("bench.py: non-finite state after the run", file=sys.stderr)
         sys.exit(1)
-    groom.destroy()
-    ctx.close()
+    if not np.array_equal(out.view(np.uint32), out2.view(np.uint32)):
+        print("bench.py: the profiled run differs from the timed run", file=sys.stderr)
+        sys.exit(1)
+    n_launch = kern["launches"]
+    step_ms, reduce_ms, mesh_ms = kern["step_ms"], kern["reduce_ms"], kern["mesh_ms"]
 
     pts = g.S * g.N
     value = pts * args.steps / el
@@ -166,6 +205,7 @@ def bench_single(args, hairsim, scenes, wl):
         b_step_kernel += 16 * g.S * (g.N - 1)
     achieved = b_step_kernel / (step_us * 1e-6) / 1e9
     whole = (b_step_kernel + b_grid + b_mesh) / ((step_us + reduce_us + mesh_us) * 1e-6) / 1e9
+    traffic, traffic_src = load_traffic(wl.name)
     res = {
         "metric": "strand-point updates/sec",
         "value": value,
@@ -182,9 +222,14 @@ def bench_single(args, hairsim, scenes, wl):
         "config": {"workload": f"{wl.name}: {g.S} strands x {g.N} points, grid {g.grid_dim}^3, "
                                f"wind + noise, orbiting sphere, friction {wl.friction}"
                                + (f", head mesh {len(wl.mesh[1])} triangles" if wl.mesh is not None else ""),
-                   "strands": g.S, "points": g.N, "grid_dim": g.grid_dim, "hair_hair": wl.hair_hair},
+                   "strands": g.S, "points": g.N, "grid_dim": g.grid_dim, "hair_hair": wl.hair_hair,
+                   "scene_steps": f"{args.warmup}..{args.warmup + args.steps - 1} timed of a {wl.steps}-step scene"
+                                  + (" (the forcing continues past its end)"
+                                     if args.warmup + args.steps > wl.steps else "")},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": load_traffic(wl.name),
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                     "kernel_timing": "HIP events around each kernel, recorded on the library's stream over the "
+                                      "same scene steps in a second run from a fresh copy of the groom",
                      "kernel": "k_step", "kernel_avg_us": step_us, "algorithmic_bytes_per_launch": b_step_kernel,
                      "reduce_kernel_avg_us": reduce_us, "reduce_algorithmic_bytes": b_grid,
                      "whole_step_achieved": whole, "whole_step_frac": whole / PEAK_HBM_GBS},
@@ -197,12 +242,15 @@ def bench_single(args, hairsim, scenes, wl):
         if mesh_us > step_us:
             # the mesh pass is the dominant kernel: it heads the roofline
             rl = res["roofline"]
+            m_traffic, m_src = load_traffic(wl.name + "-mesh")
             res["roofline"] = {"bound": "hbm", "achieved": mesh_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                               "frac": mesh_gbs / PEAK_HBM_GBS, "traffic": load_traffic(wl.name + "-mesh"),
+                               "frac": mesh_gbs / PEAK_HBM_GBS, "traffic": m_traffic, "traffic_source": m_src,
+                               "kernel_timing": rl["kernel_timing"],
                                "kernel": "k_mesh_force", "kernel_avg_us": mesh_us,
                                "algorithmic_bytes_per_launch": b_mesh,
                                "step_kernel": {k: rl[k] for k in ("achieved", "frac", "kernel_avg_us",
-                                                                  "algorithmic_bytes_per_launch", "traffic")},
+                                                                  "algorithmic_bytes_per_launch", "traffic",
+                                                                  "traffic_source")},
                                "reduce_kernel_avg_us": reduce_us, "reduce_algorithmic_bytes": b_grid,
                                "whole_step_achieved": whole, "whole_step_frac": whole / PEAK_HBM_GBS}
         else:
@@ -229,30 +277,33 @@ def bench_batched(args, hairsim, scenes, wl):
     b.set_colliders(np.stack([scenes.sphere_collider(wl.sphere(0, gi)) for gi in range(n)])[:, None])
 
     def batch_step(k):
-        b.set_time(wl.dt, float(wl.time(k % wl.steps)))
+        b.set_time(wl.dt, float(wl.time(k)))
         b.step()
 
+    args.warmup, args.steps = scene_steps(args, wl)
     # tile choice: timed on copies of the grooms (their state is untouched)
     b.set_time(wl.dt, float(wl.time(0)))
     tile, tuned_ms = ctx.autotune(b, steps=3) if not args.no_autotune else (ctx.get_tile(), None)
-    k = 0
-    for _ in range(args.warmup):
+    for k in range(args.warmup):
         batch_step(k)
-        k += 1
     ctx.sync()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for k in range(args.warmup, args.warmup + args.steps):
         batch_step(k)
-        k += 1
     ctx.sync()
     el = time.perf_counter() - t0
-    # per-kernel durations from an eager profiled pass
+    # per-kernel durations: the grooms restored to their initial state, the
+    # same scene steps again with eager launches bracketed by HIP events
+    for gr, g in zip(grooms, wl.grooms):
+        gr.write(g.strands)
     ctx.set_batch_graphs(False)
-    ctx.profile_enable(True)
-    prof_steps = min(args.steps, 20)
-    for _ in range(prof_steps):
+    for k in range(args.warmup):
         batch_step(k)
-        k += 1
+    ctx.sync()
+    ctx.profile_enable(True)
+    prof_steps = args.steps
+    for k in range(args.warmup, args.warmup + args.steps):
+        batch_step(k)
     ctx.sync()
     step_ms, reduce_ms = ctx.profile_breakdown()
     ctx.profile_enable(False)
@@ -291,6 +342,7 @@ def bench_batched(args, hairsim, scenes, wl):
     reduce_us = reduce_ms / prof_steps * 1e3
     achieved = b_step_kernel / (step_us * 1e-6) / 1e9
     whole = b_all / ((step_us + reduce_us) * 1e-6) / 1e9
+    b_traffic, b_traffic_src = load_traffic(wl.name)
     res = {
         "metric": "strand-point updates/sec",
         "value": value,
@@ -307,9 +359,13 @@ def bench_batched(args, hairsim, scenes, wl):
         "config": {"workload": f"{wl.name}: {n} grooms, {sum(g.S for g in wl.grooms)} strands, "
                                f"{pts} points, grids {wl.grooms[0].grid_dim}^3, one batched step per frame"
                                f"{'' if args.no_graphs else ' (captured graph)'}",
-                   "grooms": n, "points": pts},
+                   "grooms": n, "points": pts,
+                   "scene_steps": f"{args.warmup}..{args.warmup + args.steps - 1} timed of a {wl.steps}-step scene"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": load_traffic(wl.name),
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": b_traffic, "traffic_source": b_traffic_src,
+                     "kernel_timing": "HIP events around the step and reduce launches on the library's stream, "
+                                      "the same scene steps again with eager launches after restoring every "
+                                      "groom's initial state",
                      "kernel": "k_step_batch", "kernel_avg_us": step_us,
                      "algorithmic_bytes_per_launch": b_step_kernel, "reduce_kernel_avg_us": reduce_us,
                      "whole_step_achieved": whole, "whole_step_frac": whole / PEAK_HBM_GBS},
@@ -490,8 +546,9 @@ def bench_export(args, hairsim, scenes, samples=64):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=240)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: the rest of the workload's scene after the warm-up)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps before them (default 10)")
     ap.add_argument("--workload", default="plain-100k",
                     choices=["plain-100k", "large-1m", "plain-1k", "batched-256", "served", "export-100k"])
     ap.add_argument("--rate", type=float, default=2000.0, help="served: job arrivals per second")
@@ -521,6 +578,8 @@ def main():
         bench_served(args, hairsim, scenes)
         return
     if args.workload == "export-100k":
+        args.warmup = 5 if args.warmup is None else args.warmup
+        args.steps = 20 if args.steps is None else args.steps
         bench_export(args, hairsim, scenes)
         return
     wl = {"plain-100k": scenes.plain_100k, "large-1m": scenes.large_1m, "plain-1k": scenes.plain_1k}[args.workload]()

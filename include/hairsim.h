@@ -390,8 +390,10 @@ typedef struct hs_job {
     float* result_aos;            /* receives the final state, 48*S*N bytes     */
                                   /* (NULL: the final state is dropped)          */
 } hs_job;
-/* strands_aos, keys and result_aos are the caller's and must stay valid until
- * the job is reported done (hs_server_wait / hs_server_drain). */
+/* strands_aos, groom.seg_length, groom.wind_noise, keys and result_aos are the
+ * caller's and must stay valid until the job is reported done (hs_server_wait
+ * / hs_server_drain): the admission thread reads the first three when it
+ * uploads the job, the step loop reads the keys every step. */
 
 typedef struct hs_server_config {
     int device;
@@ -416,8 +418,12 @@ typedef struct hs_server_stats {
 typedef struct hs_server hs_server;
 
 int hs_server_create(const hs_server_config* cfg, hs_server** out);
-/* Queues a job; returns at once.  HS_ERR_BUDGET if the job alone exceeds
- * the budget or the per-step point limit. */
+/* Queues a job; returns at once.  HS_ERR_INVALID for anything hs_step_batch
+ * would reject (steps 0, dt not finite and > 0, forcing > 2, more than one
+ * collider, a mesh, a bad groom description), so a bad job never reaches a
+ * launch shared with others.  HS_ERR_BUDGET if the job alone exceeds the
+ * budget (its state, result staging and upload staging) or the per-step
+ * point limit. */
 int hs_server_submit(hs_server* srv, const hs_job* job, uint64_t* job_id);
 /* Waits until the job is done (0), or timeout_s passes (HS_ERR_STATE);
  * a negative timeout waits without limit.  HS_ERR_NOT_FOUND for an unknown

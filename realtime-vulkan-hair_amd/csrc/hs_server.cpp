@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -102,7 +103,8 @@ enum class JobState { Queued, Admitted, Active, Reading, Done, Cancelled, Failed
 struct Job {
     uint64_t id = 0;
     hs_job spec{};
-    uint64_t bytes = 0;   // groom block + read-back staging
+    uint64_t bytes = 0;   // groom block + read-back staging (held while resident)
+    uint64_t upload = 0;  // upload staging, held only while the groom is built
     uint64_t points = 0;  // S*N
     Clock::time_point t_submit;
     uint32_t done_steps = 0;
@@ -179,7 +181,8 @@ struct hs_server {
     }
     bool fits(const Job* j) const
     {
-        return resident + j->bytes <= cfg.hbm_budget && admitted_points + j->points <= cfg.max_points_per_step;
+        return resident + j->bytes + j->upload <= cfg.hbm_budget &&
+               admitted_points + j->points <= cfg.max_points_per_step;
     }
     void admit_loop();
     void step_loop();
@@ -457,8 +460,10 @@ int hs_server_submit(hs_server* s, const hs_job* job, uint64_t* job_id)
     std::string why;
     int rc = hsi::check_groom_desc(&job->groom, job->strands_aos, &why);
     if (rc != HS_OK) return rc;
-    if (job->steps == 0 || !(job->dt > 0.0f) || !job->keys || job->num_keys == 0 ||
-        job->params.num_colliders > 1 || job->params.mesh_id != 0)
+    // everything hs_step_batch would reject is rejected here: a job that
+    // failed inside a batched launch would take the launch's other jobs with it
+    if (job->steps == 0 || !(job->dt > 0.0f) || !std::isfinite(job->dt) || !job->keys || job->num_keys == 0 ||
+        job->params.num_colliders > 1 || job->params.mesh_id != 0 || job->params.forcing > 2)
         return HS_ERR_INVALID;
     auto j = std::make_unique<Job>();
     j->spec = *job;
@@ -466,7 +471,8 @@ int hs_server_submit(hs_server* s, const hs_job* job, uint64_t* job_id)
     j->bytes = hs_groom_bytes(job->groom.num_strands, job->groom.num_points, job->groom.grid_dim,
                               job->groom.flags) +
                (job->result_aos ? hsi::round_up(j->points * 48, 256) : 0);
-    if (j->bytes > s->cfg.hbm_budget || j->points > s->cfg.max_points_per_step) return HS_ERR_BUDGET;
+    j->upload = hsi::round_up(j->points * 48, 256);
+    if (j->bytes + j->upload > s->cfg.hbm_budget || j->points > s->cfg.max_points_per_step) return HS_ERR_BUDGET;
     j->t_submit = Clock::now();
     std::lock_guard<std::mutex> lk(s->m);
     j->id = s->next_id++;

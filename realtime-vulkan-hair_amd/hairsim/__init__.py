@@ -749,6 +749,12 @@ class Server:
         self._check(lib().hs_server_get_stats(self._h, C.byref(s)), "hs_server_get_stats")
         return {name: getattr(s, name) for name, _ in HsServerStats._fields_}
 
+    def error(self):
+        """(status, message) of the first failure on a server thread; (0, '') if none."""
+        msg = C.c_char_p()
+        rc = lib().hs_server_error(self._h, C.byref(msg))
+        return rc, (msg.value or b"").decode()
+
     def close(self):
         if self._h:
             lib().hs_server_destroy(self._h)

@@ -118,3 +118,30 @@ def test_server_rejects_and_cancels(ctx):
                 assert e.code == hairsim.HS_ERR_NOT_FOUND
                 cancelled += 1
         assert (done, cancelled) == (st["completed"], n)
+
+
+def test_bad_job_rejected_at_submit_others_complete(ctx):
+    """A job hs_step_batch would refuse (forcing 3, a dt that is not finite and
+    positive) is refused at submit; the jobs submitted around it run to
+    completion bit-identical to stepping them alone."""
+    jobs = _job_set(31, 3)
+    refs = [_alone(ctx, j) for j in jobs]
+    with hairsim.Server(0, hbm_budget=1 << 30, max_points_per_step=1 << 22) as srv:
+        ids = [srv.submit(jobs[0]["aos"], jobs[0]["N"], jobs[0]["steps"], DT, jobs[0]["keys"], _params(),
+                          seg_length=jobs[0]["seg"], wind_noise=jobs[0]["noise"], **GRID)[0]]
+        bad = jobs[1]
+        for dt, forcing in ((DT, 3), (float("inf"), 0), (float("nan"), 0), (-DT, 0)):
+            with pytest.raises(hairsim.HairSimError) as e:
+                srv.submit(bad["aos"], bad["N"], 5, dt, bad["keys"], _params(forcing=forcing),
+                           seg_length=bad["seg"], **GRID)
+            assert e.value.code == hairsim.HS_ERR_INVALID
+        for j in jobs[1:]:
+            ids.append(srv.submit(j["aos"], j["N"], j["steps"], DT, j["keys"], _params(), seg_length=j["seg"],
+                                  wind_noise=j["noise"], **GRID)[0])
+        for i, jid in enumerate(ids):
+            assert_bitwise(srv.wait(jid, timeout=60.0), refs[i], f"job {i}", jobs[i]["N"])
+        srv.drain()
+        code, _ = srv.error()
+        assert code == 0
+        st = srv.stats()
+    assert st["completed"] == len(jobs) and st["submitted"] == len(jobs)
