@@ -103,6 +103,7 @@ float ho_cosf(float x) { float s, c; ho_sincosf(x, &s, &c); return c; }
 
 #define R float
 #define SFX _f32
+#define FORCING_FBM(t) fbm_f32(ho_sinf(t), ho_cosf(t))
 #define SQRT sqrtf
 #define SIN ho_sinf
 #define COS ho_cosf
@@ -123,8 +124,10 @@ float ho_cosf(float x) { float s, c; ho_sincosf(x, &s, &c); return c; }
 
 float ho_fbmf(float x, float y) { return fbm_f32(x, y); }
 
+#undef FORCING_FBM
 #define R double
 #define SFX _f64
+#define FORCING_FBM(t) fbm_f32(ho_sinf((float)(t)), ho_cosf((float)(t)))
 #define SQRT sqrt
 #define SIN sin
 #define COS cos

@@ -226,7 +226,7 @@ static R F(noise2)(R px, R py)
     const R mix = a * ((R)1 - ux) + b * ux;
     return (mix + ((c - a) * uy) * ((R)1 - ux)) + ((d - b) * ux) * uy;
 }
-static R F(fbm)(R px, R py)
+__attribute__((unused)) static R F(fbm)(R px, R py)
 {
     R value = 0, amplitude = (R)0.5;
     for (int i = 0; i < 6; ++i) {
@@ -239,7 +239,11 @@ static R F(fbm)(R px, R py)
 }
 
 /* The forcing of compute.comp:150 (mode 1) or :151 (mode 2) at position x and
- * Time.totalTime t, the scalar factor on the left as the shader writes it:
+ * Time.totalTime t, the scalar factor on the left as the shader writes it
+ * (FORCING_FBM(t) is fbm(vec2(sin(t), cos(t))) evaluated in binary32 in both
+ * instantiations: the shader's hash fract(sin(.) * 43758.5453) multiplies a
+ * one-ulp difference of sin by 4.4e4, so a binary64 fbm would be another
+ * pseudo-random number, not a more accurate one):
  *   1: 10.0 * vec3(2.0*sin(t*2.0)*cos(y*10.0)*sin((y+5.0)*15.0), 0.0, -clamp(y*2.0, 0.2, 2.0))
  *   2: 7.0*fbm(vec2(sin(t), cos(t))) *
  *        vec3(2.0*sin(t*2.0)*cos(y*10.0)*sin((y+5.0)*15.0), 4.0*sin(z*5.0 + t*3.0), -0.6*(y+3.0)) */
@@ -251,7 +255,7 @@ static V3 F(forcing)(const ho_params* p, V3 x)
         const R cl = FMIN(FMAX(x.y * (R)2, (R)0.2f), (R)2);
         return F(smul)((R)10, F(v3)(sx, (R)0, -cl));
     }
-    const R amp = (R)7 * F(fbm)(SIN(t), COS(t));
+    const R amp = (R)7 * (R)FORCING_FBM(t);
     return F(smul)(amp, F(v3)(sx, (R)4 * SIN(x.z * (R)5 + t * (R)3), (R)-0.6f * (x.y + (R)3)));
 }
 
