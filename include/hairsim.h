@@ -269,12 +269,14 @@ int hs_read_grid(hs_ctx* ctx, uint32_t groom_id, hs_grid_cell* cells);
 int hs_grid_near_limit(hs_ctx* ctx, uint32_t groom_id, uint64_t* cells);
 /* Work of the last splatting step (no reference counterpart): the voxel
  * cells and boxes of the shared part of the partial-box pool (each
- * workgroup-segment's first box goes to a slot of its own and is not
- * counted) with the shared capacities, and whether anything
- * fell back to device atomics (a box larger than any LDS window, or a full
- * pool).  Zeros before the first such step.  Blocks. */
+ * workgroup-segment's own box goes to a slot of its own and is not
+ * counted) with the shared capacities, whether anything fell back to device
+ * atomics (a box larger than any LDS window, or a full pool), and how many
+ * workgroup-segments had a box larger than one window and were splatted
+ * chunk by chunk by the workgroups that finished first.  Zeros before the
+ * first such step.  Blocks. */
 typedef struct hs_grid_work_stats {
-    uint32_t pool_cells, pool_capacity, boxes, box_capacity, fallback;
+    uint32_t pool_cells, pool_capacity, boxes, box_capacity, fallback, deferred;
 } hs_grid_work_stats;
 int hs_grid_work(hs_ctx* ctx, uint32_t groom_id, hs_grid_work_stats* out);
 

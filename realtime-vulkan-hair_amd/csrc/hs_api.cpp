@@ -85,6 +85,9 @@ uint64_t carve(Groom& g, uint8_t* base)
         take(g.gw.brick_list, bricks(g.G) * 2 * hs::kBrickList * 4);
         take(g.gw.brick_nz, bricks(g.G) * 4);  // zero: every brick of the zeroed grid is empty
         take(g.gw.near_limit, 8);
+        take(g.gw.dlist, 2ull * L.nwg * L.nseg * 8);
+        take(g.gw.dclaim, 2ull * L.nwg * L.nseg * 4);
+        take(g.gw.dbox, (uint64_t)L.nwg * L.nseg * hs::kDBoxInts * 4);
         g.gw.pool_cap = (uint32_t)L.pool_cap;
         g.gw.desc_cap = (uint32_t)L.desc_cap;
         g.gw.nws = L.nwg * L.nseg;
@@ -325,6 +328,7 @@ int build_args(hs_ctx* c, Groom& g, const hs_time* t, const hs_collider* cols, u
     a.gw = g.gw;
     a.G = g.G;
     a.parity = g.parity;
+    a.tag = g.steps % 0x0fffffffu + 1u;  // 28 bits, never 0: the deferred list starts zero-filled
     a.ox = g.origin[0];
     a.oy = g.origin[1];
     a.oz = g.origin[2];
@@ -388,6 +392,7 @@ int build_args(hs_ctx* c, Groom& g, const hs_time* t, const hs_collider* cols, u
 // Host-side state after a launched step: pending friction and pool parity.
 void advance_state(Groom& g, bool splat, float friction)
 {
+    g.steps += 1;
     if (splat) {
         g.parity ^= 1u;
         g.pending = true;
@@ -1110,7 +1115,7 @@ int hs_set_tile(hs_ctx* c, const hs_tile* t)
     if (!c) return HS_ERR_INVALID;
     if (!t) return fail(c, HS_ERR_INVALID, "hs_set_tile: null tile");
     if (t->gather_cells < 64 || t->gather_cells > hs::kWinCells || t->splat_cells < 64 ||
-        t->splat_cells > hs::kWinCells / HS_VOXEL_COPIES || t->xcd_order > 1)
+        t->splat_cells > hs::kWinCells || t->xcd_order > 1)
         return fail(c, HS_ERR_INVALID, "hs_set_tile: window sizes must be in [64, 2048], xcd_order 0 or 1");
     c->tile = *t;
     return HS_OK;
@@ -1176,7 +1181,7 @@ int hs_autotune_batch(hs_ctx* c, const hs_batch_item* items, uint32_t n, uint32_
     for (uint32_t gc : gcand)
         for (uint32_t sc : scand)
             for (uint32_t xo = 0; xo < 2 && rc == HS_OK; ++xo) {
-                if (gc > hs::kWinCells || sc > hs::kWinCells / HS_VOXEL_COPIES) continue;
+                if (gc > hs::kWinCells || sc > hs::kWinCells) continue;
                 c->tile = hs_tile{gc, sc, xo};
                 rc = hs_step_batch(c, tmp.data(), n);
                 if (rc != HS_OK) break;
@@ -1320,6 +1325,7 @@ int hs_grid_work(hs_ctx* c, uint32_t id, hs_grid_work_stats* out)
     out->boxes = ctr[1];
     out->box_capacity = g->gw.desc_cap - g->gw.nws;
     out->fallback = ctr[2];
+    out->deferred = ctr[3];
     return HS_OK;
 }
 

@@ -60,6 +60,7 @@ struct Groom {
     bool pending = false;
     float pend_mu = 0.0f, pend_omu = 1.0f;
     uint32_t parity = 0;           // pool/counter half the next splat writes
+    uint32_t steps = 0;            // steps launched (the kernels' list tag)
     uint64_t bytes = 0;            // size of `block`
 };
 
@@ -139,7 +140,7 @@ struct hs_ctx {
     std::vector<BatchGraph> graph_cache;
     uint64_t batch_calls = 0;
     uint64_t graph_captures = 0;
-    hs_tile tile{hs::kWinCells, hs::kWinCells / HS_VOXEL_COPIES, 1};
+    hs_tile tile{hs::kWinCells, hs::kWinCells, 1};
 };
 
 namespace hsi {

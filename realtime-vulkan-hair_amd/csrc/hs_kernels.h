@@ -19,9 +19,6 @@ namespace hs {
 #ifndef HS_WINDOW_CELLS
 #define HS_WINDOW_CELLS 2048
 #endif
-#ifndef HS_VOXEL_COPIES
-#define HS_VOXEL_COPIES 1
-#endif
 constexpr uint32_t kW = HS_TILE_STRANDS;         // strands per step workgroup
 constexpr uint32_t kL = HS_TILE_POINTS;          // points per segment of the root-to-tip sweep
 constexpr uint32_t kWinCells = HS_WINDOW_CELLS;  // LDS window of one workgroup, 16 B entries
@@ -31,8 +28,9 @@ constexpr uint32_t kBricksPerWG = 4;
 constexpr uint32_t kMaxColliders = 8;
 constexpr uint32_t kDescInts = 8;     // partial-box descriptor: lo xyz, ext xyz, pool offset, pad
 constexpr uint32_t kBrickList = 2048;  // box indices kept per reduce brick (more: it scans every box)
-constexpr uint32_t kSlotCells = HS_WINDOW_CELLS / HS_VOXEL_COPIES;  // pool cells a workgroup-segment owns
-                                                                   // for its first box (a whole splat window)
+constexpr int kDBoxInts = 12;          // deferred segment: box lo xyz, hi xyz, chunk extents y z, counts y z, pad
+constexpr uint32_t kSlotCells = HS_WINDOW_CELLS;  // pool cells a workgroup-segment owns for its
+                                                 // first box (a whole splat window)
 
 // Device-side grid workspace of one groom (all zeroed at creation).
 struct GridWork {
@@ -40,13 +38,18 @@ struct GridWork {
     float4* grid_f;       // G^3 (float(vx), float(vy), float(vz), density > 0 ? 1/float(density) : 0)
     int4* pool;           // partial boxes written by the step kernel, [2][pool_cap]
     int* desc;            // partial-box descriptors, [2][desc_cap][kDescInts]
-    uint32_t* counters;   // [2][4]: shared cells used, shared boxes used, fallback flag, pad
+    uint32_t* counters;   // [2][4]: shared cells used, shared boxes used, fallback flag, deferred entries
     int4* overflow;       // G^3 int4 atomic fallback when the pool is full (normally zero)
     int* gbox;            // [nwg][nseg][6] voxel box each workgroup-segment touched last step
     uint32_t* brick_cnt;  // [2][nbricks] boxes listed per brick (may exceed kBrickList)
     uint32_t* brick_list; // [2][nbricks][kBrickList] indices of the boxes overlapping a brick
     uint32_t* brick_nz;   // [nbricks] 1 when the brick's grid cells are not all zero
     unsigned long long* near_limit;  // (cell, step) pairs with a component >= 2^30 in magnitude
+    // workgroup-segments whose box needs several windows, splatted chunk by
+    // chunk by any workgroup of the groom that is done with its own strands
+    unsigned long long* dlist;  // [2][nws] step tag (28 bits) | chunks (12) | workgroup-segment (24)
+    uint32_t* dclaim;           // [2][nws] chunks taken of each list entry
+    int* dbox;                  // [nws][kDBoxInts] box and chunking of a deferred segment
     // pool / descriptors per parity: first each workgroup-segment's own slot
     // (kSlotCells cells, descriptor index = its number), then a shared part
     // taken with atomics; pool_cap / desc_cap count both
@@ -87,6 +90,8 @@ struct StepArgs {
     // voxel grid (compute.comp:9-11, 204-206)
     GridWork gw;
     uint32_t G, parity;   // parity selects pool/counter half written by this launch
+    uint32_t tag;         // 1 + steps of this groom launched before, mod 2^28 - 1 (tells deferred-list
+                          // entries apart; never 0, the value of a list slot not written yet)
     float ox, oy, oz, inv_h;
     float mu_prev, omu_prev;  // friction of the previous step and 1 - friction
     // step constants
@@ -104,7 +109,7 @@ struct StepArgs {
     float f_amp, f_a2, f_t3;
     // tile choices (results do not depend on them)
     uint32_t win_gather;  // voxel cells of the gather window (<= kWinCells)
-    uint32_t win_splat;   // voxel cells of a splat window (<= kWinCells / HS_VOXEL_COPIES)
+    uint32_t win_splat;   // voxel cells of a splat window (<= kWinCells)
     uint32_t swz;         // 1: XCD-aware workgroup order
 };
 

@@ -200,18 +200,41 @@ __device__ __forceinline__ f3 gather_any(const float4* win, const Win& wd, bool 
 
 // ---------------------------------------------------------------------------
 // Splat.  Contributions int(SCALE*(w*v)) and int(SCALE*w) (compute.comp:241-248)
-// accumulate in an LDS window of kSlots copies per voxel (lane & (kSlots-1)
-// picks the copy, which spreads neighbouring lanes hitting one voxel over
-// different addresses).  A copy is 16 B:
-//   u64 (vx << 32 | density), u32 vy, u32 vz.
+// accumulate in an LDS window holding `nc` copies of every voxel (nc a power
+// of two, as many as the window's cells leave room for, at most kMaxCopies);
+// lane l adds into copy l & (nc-1).  Hair is dense: tens of lanes of one
+// wavefront land in the same voxel block, and same-address LDS atomics
+// serialise, so a small box is spread over copies that the dump sums.
+// A copy is 16 B:  u64 (vx << 32 | density), u32 vy, u32 vz.
 // Density addends are in [0, 1e6] and one window holds at most kW*(kL+1)
 // points, so the low word never carries into vx: both sums are exact mod 2^32.
 // ---------------------------------------------------------------------------
-constexpr int kSlots = HS_VOXEL_COPIES;
-static_assert((kSlots & (kSlots - 1)) == 0, "copies per voxel: a power of two");
-constexpr int kSplatCells = (int)kWinCells / kSlots;  // voxels per splat window
+constexpr int kMaxCopies = 16;
+// segments of the longest strand (256 points): the most deferred-list
+// entries one workgroup can publish in a step
+#define HS_MAX_OWN_DEFERRED ((256 - 1 + HS_TILE_POINTS - 1) / HS_TILE_POINTS)
 
-__device__ __forceinline__ void splat_cell(unsigned char* cellp, float w, f3 v)
+// Where a lane's splat goes: its copy and the corner it starts its 2x2x2
+// block at (lanes that share a copy start at different corners).  Copy k of
+// window voxel w is entry w * nc + k.  (Copy-major orders and separate
+// planes per component were measured no faster.)
+struct SplatLane {
+    int slot, rot, nc;
+};
+
+// Copies of each voxel a window of `cells` voxels gets.
+__device__ __forceinline__ SplatLane splat_lane(int cells)
+{
+    int nc = 1;
+    while (nc < kMaxCopies && cells * nc * 2 <= (int)kWinCells) nc *= 2;
+    const int lane = (int)threadIdx.x;
+    return SplatLane{lane & (nc - 1), (lane / nc) & 7, nc};
+}
+
+// entry index of window voxel w in this lane's copy
+__device__ __forceinline__ int splat_entry(int w, const SplatLane& sl) { return w * sl.nc + sl.slot; }
+
+__device__ __forceinline__ void splat_cell(unsigned char* win, int e, float w, f3 v)
 {
     const float SCALE = 1000000.0f;
     const f3 wv = smul(w, v);
@@ -219,63 +242,51 @@ __device__ __forceinline__ void splat_cell(unsigned char* cellp, float w, f3 v)
     const uint32_t iy = (uint32_t)sat_i32(SCALE * wv.y);
     const uint32_t iz = (uint32_t)sat_i32(SCALE * wv.z);
     const uint32_t id = (uint32_t)sat_i32(SCALE * w);
+    unsigned char* cellp = win + 16 * e;
     atomicAdd(reinterpret_cast<unsigned long long*>(cellp), ((unsigned long long)ix << 32) | id);
     atomicAdd(reinterpret_cast<uint32_t*>(cellp + 8), iy);
     atomicAdd(reinterpret_cast<uint32_t*>(cellp + 12), iz);
 }
 
-// byte address of voxel (a,b,c)'s copy for this lane
-__device__ __forceinline__ unsigned char* splat_addr(int4* win, const Win& wd, int a, int b, int c, int slot)
+// Sum of window voxel w's copies as a GridCell (vx, vy, vz, density).
+__device__ __forceinline__ int4 splat_sum(const int4* win, int w, const SplatLane& sl)
 {
-    return reinterpret_cast<unsigned char*>(win) + 16 * (win_index(wd, a, b, c) * kSlots + slot);
+    uint64_t A = 0;
+    uint32_t sy = 0, sz = 0;
+    for (int k = 0; k < sl.nc; ++k) {
+        const int4 raw = win[w * sl.nc + k];
+        A += ((uint64_t)(uint32_t)raw.y << 32) | (uint32_t)raw.x;
+        sy += (uint32_t)raw.z;
+        sz += (uint32_t)raw.w;
+    }
+    return make_int4((int)(uint32_t)(A >> 32), (int)sy, (int)sz, (int)(uint32_t)A);
 }
 
-#ifndef HS_SPLAT_ROTATE
-#define HS_SPLAT_ROTATE 1
-#endif
-
 // Full 2x2x2 block inside the LDS window.
-__device__ __forceinline__ void splat_block(int4* win, const Win& wd, const Rng& r, f3 q, f3 v, int slot)
+__device__ __forceinline__ void splat_block(int4* win, const Win& wd, const Rng& r, f3 q, f3 v, const SplatLane& sl)
 {
     const float wx0 = wgt(q.x, r.x0), wx1 = wgt(q.x, r.x1);
     const float wy0 = wgt(q.y, r.y0), wy1 = wgt(q.y, r.y1);
     const float wz0 = wgt(q.z, r.z0), wz1 = wgt(q.z, r.z1);
-    const int sx = 16 * kSlots, sy = sx * wd.ex, sz = sy * wd.ey;
-    unsigned char* b = splat_addr(win, wd, r.x0, r.y0, r.z0, slot);
-#if HS_SPLAT_ROTATE
-    // Neighbouring strands share voxels: if every lane added corner 0 first,
-    // up to ~30 lanes of a wavefront would hit one LDS address per atomic.
-    // Lane l visits the corners starting at l % 8, so lanes of one voxel
-    // block hit 8 different cells per instruction.  Integer adds commute and
-    // each corner's weight is still (wx * wy) * wz: the sums are unchanged.
-    const int rot = (int)(threadIdx.x & 7u);
+    const int sx = sl.nc, sy = sx * wd.ex, sz = sy * wd.ey;
+    const int e0 = splat_entry(win_index(wd, r.x0, r.y0, r.z0), sl);
+    unsigned char* wb = reinterpret_cast<unsigned char*>(win);
+    // Lanes that share a copy start at different corners, so lanes of one
+    // voxel block hit different cells per instruction.  Integer adds commute
+    // and each corner's weight is still (wx * wy) * wz: the sums are unchanged.
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        const int k = (i + rot) & 7;
+        const int k = (i + sl.rot) & 7;
         const bool dx = (k & 1) != 0, dy = (k & 2) != 0, dz = (k & 4) != 0;
         const float w = ((dx ? wx1 : wx0) * (dy ? wy1 : wy0)) * (dz ? wz1 : wz0);
-        splat_cell(b + (dx ? sx : 0) + (dy ? sy : 0) + (dz ? sz : 0), w, v);
+        splat_cell(wb, e0 + (dx ? sx : 0) + (dy ? sy : 0) + (dz ? sz : 0), w, v);
     }
-    return;
-#endif
-    float wxy = wx0 * wy0;
-    splat_cell(b, wxy * wz0, v);
-    splat_cell(b + sz, wxy * wz1, v);
-    wxy = wx0 * wy1;
-    splat_cell(b + sy, wxy * wz0, v);
-    splat_cell(b + sy + sz, wxy * wz1, v);
-    wxy = wx1 * wy0;
-    splat_cell(b + sx, wxy * wz0, v);
-    splat_cell(b + sx + sz, wxy * wz1, v);
-    wxy = wx1 * wy1;
-    splat_cell(b + sx + sy, wxy * wz0, v);
-    splat_cell(b + sx + sy + sz, wxy * wz1, v);
 }
 
 // Any non-empty block: into the LDS window when `lds`, else device atomics
 // into the global overflow grid (when no window fits).
-__device__ __forceinline__ void splat_any(int4* win, const Win& wd, bool lds, int slot, int4* __restrict__ g,
-                                          const Rng& r, f3 q, f3 v, int G)
+__device__ __forceinline__ void splat_any(int4* win, const Win& wd, bool lds, const SplatLane& sl,
+                                          int4* __restrict__ g, const Rng& r, f3 q, f3 v, int G)
 {
     const float SCALE = 1000000.0f;
 #pragma unroll 1
@@ -288,7 +299,7 @@ __device__ __forceinline__ void splat_any(int4* win, const Win& wd, bool lds, in
             for (int c = r.z0; c <= r.z1; ++c) {
                 const float w = wxy * wgt(q.z, c);
                 if (lds) {
-                    splat_cell(splat_addr(win, wd, a, b, c, slot), w, v);
+                    splat_cell(reinterpret_cast<unsigned char*>(win), splat_entry(win_index(wd, a, b, c), sl), w, v);
                 } else {
                     const f3 wv = smul(w, v);
                     int* cell = reinterpret_cast<int*>(g + ((size_t)a + (size_t)b * G + (size_t)c * G * G));
@@ -624,7 +635,7 @@ struct Plan {
     bool ok;          // false: no chunking fits (or empty box)
 };
 
-__device__ __forceinline__ Plan plan_box(const Box& b, int cap)
+__device__ __forceinline__ Plan plan_box(const Box& b, int cap, bool splat_plan = false)
 {
     Plan p;
     if (box_is_empty(b)) {
@@ -641,6 +652,11 @@ __device__ __forceinline__ Plan plan_box(const Box& b, int cap)
     if (p.ex * p.ey * p.ez <= cap) {
         p.cy = p.ey;
         p.cz = p.ez;
+#ifdef HS_NO_CHUNKS
+    } else if (splat_plan) {
+        p.ok = false;
+        p.cy = p.cz = 2;
+#endif
     } else if (p.ex * p.ey * 2 <= cap) {
         p.cy = p.ey;
         p.cz = cap / (p.ex * p.ey);
@@ -669,6 +685,264 @@ __device__ __forceinline__ Win chunk_of(const Box& b, const Plan& p, int iy, int
 }
 
 // ---------------------------------------------------------------------------
+// Partial boxes: an LDS window is summed over its copies and dumped into the
+// pool, and its box is listed with every reduce brick it overlaps.
+// ---------------------------------------------------------------------------
+// `own` >= 0: the dump goes to that workgroup-segment's own pool slot and
+// descriptor (no allocator); otherwise to the shared part, taken with atomics.
+// Every thread of the workgroup calls this after the window is complete.
+__device__ __forceinline__ void dump_window(const StepArgs& a, const Win& cw, int cells, const SplatLane& sl, int own, int4* win,
+                            uint32_t* bcast)
+{
+    uint32_t* ctr = a.gw.counters + a.parity * 4;
+    if (threadIdx.x == 0) {
+        uint32_t off, di;
+        if (own >= 0) {
+            off = (uint32_t)own * kSlotCells;
+            di = (uint32_t)own;
+        } else {
+            off = a.gw.nws * kSlotCells + atomicAdd(ctr + 0, (uint32_t)cells);
+            di = a.gw.nws + atomicAdd(ctr + 1, 1u);
+        }
+        const bool fits = (uint64_t)off + cells <= a.gw.pool_cap && di < a.gw.desc_cap;
+        if (di < a.gw.desc_cap) {
+            int* d = a.gw.desc + ((size_t)a.parity * a.gw.desc_cap + di) * kDescInts;
+            d[0] = cw.x0;
+            d[1] = cw.y0;
+            d[2] = cw.z0;
+            d[3] = fits ? cw.ex : 0;  // ex == 0: empty descriptor
+            d[4] = cw.ey;
+            d[5] = cw.ez;
+            d[6] = (int)off;
+            d[7] = 0;
+        }
+        if (!fits) atomicOr(ctr + 2, 1u);
+        bcast[0] = off;
+        bcast[1] = fits ? 1u : 0u;
+        bcast[2] = di;
+    }
+    __syncthreads();
+    const uint32_t off = bcast[0];
+    const bool fits = bcast[1] != 0;
+    const int G = (int)a.G;
+    if (fits) {
+        // list the box with every brick it overlaps (one device atomic each, in parallel)
+        const uint32_t di = bcast[2];
+        const int bx0 = cw.x0 / (int)kBX, by0 = cw.y0 / (int)kBY, bz0 = cw.z0 / (int)kBZ;
+        const int nbx = (cw.x0 + cw.ex - 1) / (int)kBX - bx0 + 1;
+        const int nby = (cw.y0 + cw.ey - 1) / (int)kBY - by0 + 1;
+        const int nbz = (cw.z0 + cw.ez - 1) / (int)kBZ - bz0 + 1;
+        const uint32_t nbr = a.gw.nbx * a.gw.nby * a.gw.nbz;
+        for (int t = threadIdx.x; t < nbx * nby * nbz; t += kW) {
+            const uint32_t b = ((uint32_t)(bz0 + t / (nbx * nby)) * a.gw.nby + (uint32_t)(by0 + (t / nbx) % nby)) *
+                                   a.gw.nbx +
+                               (uint32_t)(bx0 + t % nbx);
+            const uint32_t at = atomicAdd(a.gw.brick_cnt + a.parity * nbr + b, 1u);
+            if (at < kBrickList) a.gw.brick_list[((size_t)a.parity * nbr + b) * kBrickList + at] = di;
+        }
+    }
+    int4* dst = a.gw.pool + (size_t)a.parity * a.gw.pool_cap + off;
+    for (int c = threadIdx.x; c < cells; c += kW) {
+        const int4 cell = splat_sum(win, c, sl);
+        if (fits) {
+            dst[c] = cell;
+        } else if (cell.x | cell.y | cell.z | cell.w) {
+            // no room in the pool: device atomics into the overflow grid
+            const int ax = c % cw.ex;
+            const int rest = c / cw.ex;
+            const size_t gi = (size_t)(cw.x0 + ax) + (size_t)(cw.y0 + rest % cw.ey) * G +
+                              (size_t)(cw.z0 + rest / cw.ey) * G * G;
+            int* gcell = reinterpret_cast<int*>(a.gw.overflow + gi);
+            atomicAdd(gcell + 0, cell.x);
+            atomicAdd(gcell + 1, cell.y);
+            atomicAdd(gcell + 2, cell.z);
+            atomicAdd(gcell + 3, cell.w);
+        }
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Deferred chunks.  A workgroup-segment whose voxel box needs several windows
+// records its box and chunking (dbox[ws]), then appends ws to the step's list
+// (dlist[parity][k] = step tag, chunk count, ws; counters[parity][3] entries).  Chunk c of
+// list entry k is taken by whoever first raises dclaim[parity][k] past c, and
+// splatted from the segment's stored positions and velocities.  Each point
+// goes to exactly one chunk: the one whose y/z range starts at or before its
+// block and reaches past it (chunk starts step by window size - 1, so that is
+// the quotient below).  Integer voxel sums commute, so the grid equals the
+// one the segment would have splatted itself.
+// Every workgroup of the groom, when done with its own strands, reads the
+// list once and takes any chunk still free; the list's owners additionally
+// take all chunks of their own entries that are left, so every chunk is done
+// before the last owner finishes, without any workgroup waiting on another.
+// `tag` (a per-groom step count) tells this step's entries from older ones
+// that share the slot.
+// ---------------------------------------------------------------------------
+// A list entry: step tag (28 bits) | chunk count (12 bits) | workgroup-segment (24 bits).
+constexpr uint32_t kMaxDeferredChunks = 4095;
+__device__ __forceinline__ unsigned long long dentry(uint32_t tag, uint32_t n, uint32_t ws)
+{
+    return ((unsigned long long)(tag & 0x0fffffffu) << 36) | ((unsigned long long)n << 24) | ws;
+}
+__device__ __forceinline__ bool dentry_is(unsigned long long e, uint32_t tag) { return (uint32_t)(e >> 36) == (tag & 0x0fffffffu); }
+__device__ __forceinline__ uint32_t dentry_n(unsigned long long e) { return (uint32_t)(e >> 24) & 0xfffu; }
+__device__ __forceinline__ uint32_t dentry_ws(unsigned long long e) { return (uint32_t)e & 0xffffffu; }
+
+__device__ __forceinline__ uint32_t publish_deferred(const StepArgs& a, uint32_t ws, const Box& box, const Plan& pl)
+{
+    int* db = a.gw.dbox + (size_t)ws * kDBoxInts;
+    for (int k = 0; k < 3; ++k) {
+        db[k] = box.lo[k];
+        db[3 + k] = box.hi[k];
+    }
+    db[6] = pl.cy;
+    db[7] = pl.cz;
+    db[8] = pl.ny;
+    db[9] = pl.nz;
+    __threadfence();  // the plan (and the segment's stored state) before the list entry
+    uint32_t* ctr = a.gw.counters + a.parity * 4;
+    const uint32_t k = atomicAdd(ctr + 3, 1u);
+    __hip_atomic_store(a.gw.dlist + (size_t)a.parity * a.gw.nws + k, dentry(a.tag, (uint32_t)(pl.ny * pl.nz), ws),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return k;
+}
+
+// Chunk c of the deferred workgroup-segment ws (plan at db): zero the window,
+// splat the segment's finished points whose blocks fall in this chunk, dump.
+__device__ __forceinline__ void splat_deferred_chunk(const StepArgs& a, uint32_t ws, const int* db, int c, int4* win, uint32_t* bcast)
+{
+    Box box;
+    for (int k = 0; k < 3; ++k) {
+        box.lo[k] = db[k];
+        box.hi[k] = db[3 + k];
+    }
+    Plan pl;
+    pl.ex = box.hi[0] - box.lo[0] + 1;
+    pl.ey = box.hi[1] - box.lo[1] + 1;
+    pl.ez = box.hi[2] - box.lo[2] + 1;
+    pl.cy = db[6];
+    pl.cz = db[7];
+    pl.ny = db[8];
+    pl.nz = db[9];
+    pl.ok = true;
+    const int iy = c % pl.ny, iz = c / pl.ny;
+    const Win cw = chunk_of(box, pl, iy, iz);
+    const int cells = cw.ex * cw.ey * cw.ez;
+    const SplatLane sl = splat_lane(cells);
+    for (int e = threadIdx.x; e < cells * sl.nc; e += kW) win[e] = make_int4(0, 0, 0, 0);
+    __syncthreads();
+    const uint32_t wgo = ws / a.nseg, sg = ws % a.nseg;
+    const uint32_t s = wgo * kW + threadIdx.x;
+    const uint32_t N = a.N, Sp = a.Sp;
+    const uint32_t f = 1 + sg * kL;
+    const uint32_t cnt = min(kL, N - f);
+    const bool last_seg = f + cnt == N;
+    if (s < a.S) {
+        const uint32_t P4 = N * Sp * 4u, P8 = N * Sp * 8u;
+        const f3 origin = mk(a.ox, a.oy, a.oz);
+        // finished points of the segment: f-1 (carried), f..f+cnt-2, and the
+        // tip; four at a time, their loads in flight together
+        const uint32_t i0 = f >= 2 ? f - 1 : f;
+        const uint32_t i1 = last_seg ? f + cnt - 1 : f + cnt - 2;
+        constexpr uint32_t kGrp = 4;
+#pragma unroll 1
+        for (uint32_t g0 = i0; g0 <= i1; g0 += kGrp) {
+        f3 XP[kGrp], VP[kGrp];
+#pragma unroll
+        for (uint32_t t = 0; t < kGrp; ++t) {
+            if (g0 + t > i1) continue;
+            const uint32_t bo = ((g0 + t) * Sp + s) * 4u;
+            XP[t] = ld3(a.pos, bo, P4, P8);
+            VP[t] = ld3(a.vel, bo, P4, P8);
+        }
+#pragma unroll
+        for (uint32_t t = 0; t < kGrp; ++t) {
+            if (g0 + t > i1) continue;
+            const f3 x = XP[t];
+            const f3 v = VP[t];
+            const f3 q = grid_coord(x, origin, a.inv_h);
+            const Rng rg = point_range(q, (int)a.G);
+            if (rng_empty(rg)) continue;
+            const int jy = pl.ny > 1 ? min(pl.ny - 1, (rg.y0 - box.lo[1]) / (pl.cy - 1)) : 0;
+            const int jz = pl.nz > 1 ? min(pl.nz - 1, (rg.z0 - box.lo[2]) / (pl.cz - 1)) : 0;
+            if (jy != iy || jz != iz) continue;
+            if (rng_full(rg))
+                splat_block(win, cw, rg, q, v, sl);
+            else
+                splat_any(win, cw, true, sl, a.gw.overflow, rg, q, v, (int)a.G);
+        }
+        }
+    }
+    __syncthreads();
+    dump_window(a, cw, cells, sl, -1, win, bcast);
+}
+
+// Run by every workgroup at its end (all threads): help with, or (for its own
+// entries) finish, the step's deferred chunks.  The list is read with one
+// thread per entry, so a workgroup with nothing to take is done after one
+// round trip.  Workgroups finish at about the same time, so a helper starts
+// at an entry picked by its index, takes at most one chunk and gives up after
+// two entries with none left; the owners then take every chunk of their own
+// entries that is still free.
+__device__ __forceinline__ bool take_chunk(const StepArgs& a, uint32_t kk, int4* win, uint32_t* bcast)
+{
+    const unsigned long long* list = a.gw.dlist + (size_t)a.parity * a.gw.nws;
+    uint32_t* claim = a.gw.dclaim + (size_t)a.parity * a.gw.nws;
+    if (threadIdx.x == 0) {
+        const unsigned long long e = __hip_atomic_load(list + kk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        bcast[0] = dentry_ws(e);
+        bcast[1] = atomicAdd(claim + kk, 1u);
+        bcast[2] = dentry_n(e);
+        __threadfence();  // the plan and the owner's stored points after its list entry
+    }
+    __syncthreads();
+    const uint32_t ws = bcast[0], c = bcast[1], n = bcast[2];
+    __syncthreads();
+    if (c >= n) return false;
+    splat_deferred_chunk(a, ws, a.gw.dbox + (size_t)ws * kDBoxInts, (int)c, win, bcast);
+    return true;
+}
+
+__device__ __forceinline__ void deferred_chunks(const StepArgs& a, uint32_t wg, int4* win, uint32_t* bcast,
+                                                uint32_t* want, const uint32_t* own_k, const uint32_t* n_own_p)
+{
+    uint32_t* ctr = a.gw.counters + a.parity * 4;
+    __syncthreads();
+    if (threadIdx.x == 0) bcast[0] = __hip_atomic_load(ctr + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const uint32_t nd = bcast[0];
+    const uint32_t n_own = *n_own_p;  // written by thread 0 before the barrier above
+    if (nd == 0) return;
+    const unsigned long long* list = a.gw.dlist + (size_t)a.parity * a.gw.nws;
+    const uint32_t* claim = a.gw.dclaim + (size_t)a.parity * a.gw.nws;
+    // want[0]: entries of other workgroups with chunks left, their indices in
+    // want[1..]; at most kW - 1 per pass
+    for (uint32_t base = 0; base < nd; base += kW - 1) {
+        if (threadIdx.x == 0) want[0] = 0;
+        __syncthreads();
+        const uint32_t k = base + threadIdx.x;
+        if (threadIdx.x < kW - 1 && k < nd) {
+            const unsigned long long e = __hip_atomic_load(list + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t taken = __hip_atomic_load(claim + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (dentry_is(e, a.tag) && dentry_ws(e) / a.nseg != wg && taken < dentry_n(e))
+                want[1 + atomicAdd(want, 1u)] = k;
+        }
+        __syncthreads();
+        const uint32_t nw = want[0];
+        // help with one chunk: try two entries, starting at one picked by this
+        // workgroup's index
+        for (uint32_t tried = 0, j = wg % max(nw, 1u); tried < min(nw, 2u); ++tried, j = (j + 1) % nw)
+            if (take_chunk(a, want[1 + j], win, bcast)) break;
+        __syncthreads();
+    }
+    // its own entries (recorded when published): everything left
+    for (uint32_t j = 0; j < n_own; ++j)
+        while (take_chunk(a, own_k[j], win, bcast)) {
+        }
+}
+
+// ---------------------------------------------------------------------------
 // The step kernel
 // ---------------------------------------------------------------------------
 // XCD-aware order: dispatch places consecutive blockIdx on different XCDs
@@ -685,12 +959,14 @@ template <bool GATHER, bool SPLAT, int COL>
 __device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
 {
     __shared__ __align__(16) int4 win[kWinCells];
-    __shared__ int red[kWaves * 6];
+    __shared__ int red[kW > kWaves * 6 ? kW : kWaves * 6];  // box reductions; the deferred-list scan
     __shared__ uint32_t bcast[3];
+    __shared__ uint32_t own_k[HS_MAX_OWN_DEFERRED];  // deferred-list entries this workgroup published
+    __shared__ uint32_t n_own[1];
+    if (SPLAT && threadIdx.x == 0) n_own[0] = 0;
 
     const uint32_t s = wg * kW + threadIdx.x;
     const bool active = s < a.S;
-    const int slot = threadIdx.x & (kSlots - 1);
     const uint32_t Sp = a.Sp;
     const uint32_t N = a.N;
     const uint32_t plane = N * Sp;
@@ -772,29 +1048,49 @@ __device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
         }
 
         // ------- P: friction, force and predicted position of every point -------
+        // Previous step's phase 3 (compute.comp:257-298) for each point: from
+        // the LDS window, or (a workgroup whose box exceeds any window)
+        // straight from the float grid, the 8 cells of a block loaded together.
+        auto predict = [&](uint32_t t, f3 gv) {
+            const f3 x = X[t];
+            f3 u = U[t];
+            if (GATHER) u = friction(u, gv, a.mu_prev, a.omu_prev);
+            // predict (compute.comp:187)
+            float4 mf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (COL & kColMesh) mf = a.mesh_f[(size_t)(f + t) * Sp + s];
+            const f3 fc = force_at<COL>(x, fs, a, mf);
+            U[t] = add(add(x, smul(a.dt, u)), smul(a.dt2, fc));
+        };
         if (active) {
+            if (!GATHER || gwin_ok) {
 #pragma unroll
-            for (uint32_t t = 0; t < kL; ++t) {
-                if (t >= cnt) continue;
-                const f3 x = X[t];
-                f3 u = U[t];
-                if (GATHER) {
-                    // previous step's phase 3 (compute.comp:257-298) for this point
-                    const f3 q = grid_coord(x, origin, a.inv_h);
+                for (uint32_t t = 0; t < kL; ++t) {
+                    if (t >= cnt) continue;
+                    f3 gv = mk(0.0f, 0.0f, 0.0f);
+                    if (GATHER) {
+                        const f3 q = grid_coord(X[t], origin, a.inv_h);
+                        const Rng rg = point_range(q, G);
+                        const bool inw = in_win(rg, gwin);
+                        if (inw && rng_full(rg))
+                            gv = gather_block(reinterpret_cast<const float4*>(win), gwin, rg, q);
+                        else if (!rng_empty(rg))
+                            gv = gather_any(reinterpret_cast<const float4*>(win), gwin, inw, a.gw.grid_f, rg, q, G);
+                    }
+                    predict(t, gv);
+                }
+            } else {
+#pragma unroll
+                for (uint32_t t = 0; t < kL; ++t) {
+                    if (t >= cnt) continue;
+                    const f3 q = grid_coord(X[t], origin, a.inv_h);
                     const Rng rg = point_range(q, G);
                     f3 gv = mk(0.0f, 0.0f, 0.0f);
-                    const bool inw = gwin_ok && in_win(rg, gwin);
-                    if (inw && rng_full(rg))
-                        gv = gather_block(reinterpret_cast<const float4*>(win), gwin, rg, q);
+                    if (rng_full(rg))
+                        gv = gather_block(a.gw.grid_f, Win{0, 0, 0, G, G, G}, rg, q);  // the whole grid as the window
                     else if (!rng_empty(rg))
-                        gv = gather_any(reinterpret_cast<const float4*>(win), gwin, inw, a.gw.grid_f, rg, q, G);
-                    u = friction(u, gv, a.mu_prev, a.omu_prev);
+                        gv = gather_any_src<false>(nullptr, gwin, a.gw.grid_f, rg, q, G);
+                    predict(t, gv);
                 }
-                // predict (compute.comp:187)
-                float4 mf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                if (COL & kColMesh) mf = a.mesh_f[(size_t)(f + t) * Sp + s];
-                const f3 fc = force_at<COL>(x, fs, a, mf);
-                U[t] = add(add(x, smul(a.dt, u)), smul(a.dt2, fc));
             }
         }
         if (GATHER && gwin_ok) __syncthreads();  // the window is reused below
@@ -864,23 +1160,25 @@ __device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
                 }
             }
             const Box box = block_box(sbox, red);
-            const Plan pl = plan_box(box, min((int)a.win_splat, kSplatCells));
+            const Plan pl = plan_box(box, min((int)a.win_splat, (int)kWinCells), true);
             uint32_t* ctr = a.gw.counters + a.parity * 4;
-            if (!box_is_empty(box) && !pl.ok) {
-                // no window fits: device atomics into the overflow grid
+            if (!box_is_empty(box) && (!pl.ok || pl.ny * pl.nz > (int)kMaxDeferredChunks)) {
+                // no window fits (or more chunks than a list entry counts):
+                // device atomics into the overflow grid
                 const Win none{0, 0, 0, 0, 0, 0};
+                const SplatLane sl = splat_lane((int)kWinCells);
                 if (active) {
                     if (f >= 2) {
                         const f3 q = grid_coord(cp, origin, a.inv_h);
                         const Rng rg = point_range(q, G);
-                        if (!rng_empty(rg)) splat_any(win, none, false, slot, a.gw.overflow, rg, q, cv, G);
+                        if (!rng_empty(rg)) splat_any(win, none, false, sl, a.gw.overflow, rg, q, cv, G);
                     }
 #pragma unroll
                     for (uint32_t t = 0; t < kL; ++t) {
                         if (!(t + 1 < cnt || (t + 1 == cnt && last_seg))) continue;
                         const f3 q = grid_coord(X[t], origin, a.inv_h);
                         const Rng rg = point_range(q, G);
-                        if (!rng_empty(rg)) splat_any(win, none, false, slot, a.gw.overflow, rg, q, U[t], G);
+                        if (!rng_empty(rg)) splat_any(win, none, false, sl, a.gw.overflow, rg, q, U[t], G);
                     }
                 }
                 if (threadIdx.x == 0) atomicOr(ctr + 2, 1u);
@@ -888,146 +1186,49 @@ __device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
             // this workgroup-segment's own pool slot and descriptor
             const uint32_t ws = wg * a.nseg + sg;
             bool slot_used = false;
-            if (pl.ok) {
-                uint32_t done = 0;  // bit kL: carried point
-                for (int iz = 0; iz < pl.nz; ++iz)
-                    for (int iy = 0; iy < pl.ny; ++iy) {
-                        const Win cw = chunk_of(box, pl, iy, iz);
-                        const int cells = cw.ex * cw.ey * cw.ez;
-                        if (pl.ny * pl.nz > 1) {
-                            // several chunks: skip the ones no remaining point falls in
-                            bool mine = false;
-                            if (active) {
-                                if (f >= 2 && !((done >> kL) & 1u)) {
-                                    const Rng rg = point_range(grid_coord(cp, origin, a.inv_h), G);
-                                    mine |= !rng_empty(rg) && in_win(rg, cw);
-                                }
-#pragma unroll
-                                for (uint32_t t = 0; t < kL; ++t) {
-                                    if (!(t + 1 < cnt || (t + 1 == cnt && last_seg)) || ((done >> t) & 1u)) continue;
-                                    f3 xp = X[t];
-                                    uint32_t rp = RP[t];
-                                    asm volatile("" : "+v"(xp.x), "+v"(xp.y), "+v"(xp.z), "+v"(rp));
-                                    const Rng rg = rp ? unpack_block(rp) : point_range(grid_coord(xp, origin, a.inv_h), G);
-                                    mine |= !rng_empty(rg) && in_win(rg, cw);
-                                }
-                            }
-                            if (!__syncthreads_or(mine)) continue;
+            if (pl.ok && pl.ny * pl.nz > 1 && pl.ny * pl.nz <= (int)kMaxDeferredChunks) {
+                // A box larger than one window: splatting it here would take
+                // one pass over the segment's points per chunk.  Publish the
+                // plan instead; the segment's positions and final velocities
+                // are already stored, so workgroups that have finished their
+                // own strands can take chunks in parallel, and this workgroup
+                // takes what is left once it is done (deferred_chunks).
+                __threadfence();  // this thread's stored positions and velocities, before the entry
+                __syncthreads();
+                if (threadIdx.x == 0) own_k[n_own[0]++] = publish_deferred(a, ws, box, pl);
+            } else if (pl.ok && pl.ny * pl.nz == 1) {
+                // the whole box in one window
+                const Win cw = chunk_of(box, pl, 0, 0);
+                const int cells = cw.ex * cw.ey * cw.ez;
+                const SplatLane sl = splat_lane(cells);
+                for (int c = threadIdx.x; c < cells * sl.nc; c += kW) win[c] = make_int4(0, 0, 0, 0);
+                __syncthreads();
+                if (active) {
+                    if (f >= 2) {
+                        const f3 q = grid_coord(cp, origin, a.inv_h);
+                        const Rng rg = point_range(q, G);
+                        if (!rng_empty(rg)) {
+                            if (rng_full(rg))
+                                splat_block(win, cw, rg, q, cv, sl);
+                            else
+                                splat_any(win, cw, true, sl, a.gw.overflow, rg, q, cv, G);
                         }
-                        for (int c = threadIdx.x; c < cells * kSlots; c += kW) win[c] = make_int4(0, 0, 0, 0);
-                        __syncthreads();
-                        if (active) {
-                            if (f >= 2 && !((done >> kL) & 1u)) {
-                                const f3 q = grid_coord(cp, origin, a.inv_h);
-                                const Rng rg = point_range(q, G);
-                                if (!rng_empty(rg) && in_win(rg, cw)) {
-                                    if (rng_full(rg))
-                                        splat_block(win, cw, rg, q, cv, slot);
-                                    else
-                                        splat_any(win, cw, true, slot, a.gw.overflow, rg, q, cv, G);
-                                    done |= 1u << kL;
-                                }
-                            }
-#pragma unroll
-                            for (uint32_t t = 0; t < kL; ++t) {
-                                if (!(t + 1 < cnt || (t + 1 == cnt && last_seg))) continue;
-                                if ((done >> t) & 1u) continue;
-                                // opaque copy: keeps the per-point voxel math inside the
-                                // chunk loop instead of hoisting 16 points' worth of it
-                                f3 xp = X[t];
-                                uint32_t rp = RP[t];
-                                asm volatile("" : "+v"(xp.x), "+v"(xp.y), "+v"(xp.z), "+v"(rp));
-                                const f3 q = grid_coord(xp, origin, a.inv_h);
-                                const Rng rg = rp ? unpack_block(rp) : point_range(q, G);
-                                if (rng_empty(rg) || !in_win(rg, cw)) continue;
-                                if (rng_full(rg))
-                                    splat_block(win, cw, rg, q, U[t], slot);
-                                else
-                                    splat_any(win, cw, true, slot, a.gw.overflow, rg, q, U[t], G);
-                                done |= 1u << t;
-                            }
-                        }
-                        __syncthreads();
-#ifndef HS_OWN_SLOTS
-#define HS_OWN_SLOTS 1
-#endif
-                        const bool own = HS_OWN_SLOTS && !slot_used && cells <= (int)kSlotCells;
-                        slot_used |= own;
-                        if (threadIdx.x == 0) {
-                            uint32_t off, di;
-                            if (own) {
-                                // no allocator: the slot is this workgroup-segment's
-                                off = ws * kSlotCells;
-                                di = ws;
-                            } else {
-                                off = a.gw.nws * kSlotCells + atomicAdd(ctr + 0, (uint32_t)cells);
-                                di = a.gw.nws + atomicAdd(ctr + 1, 1u);
-                            }
-                            const bool fits = (uint64_t)off + cells <= a.gw.pool_cap && di < a.gw.desc_cap;
-                            if (di < a.gw.desc_cap) {
-                                int* d = a.gw.desc + ((size_t)a.parity * a.gw.desc_cap + di) * kDescInts;
-                                d[0] = cw.x0;
-                                d[1] = cw.y0;
-                                d[2] = cw.z0;
-                                d[3] = fits ? cw.ex : 0;  // ex == 0: empty descriptor
-                                d[4] = cw.ey;
-                                d[5] = cw.ez;
-                                d[6] = (int)off;
-                                d[7] = 0;
-                            }
-                            if (!fits) atomicOr(ctr + 2, 1u);
-                            bcast[0] = off;
-                            bcast[1] = fits ? 1u : 0u;
-                            bcast[2] = di;
-                        }
-                        __syncthreads();
-                        const uint32_t off = bcast[0];
-                        const bool fits = bcast[1] != 0;
-                        if (fits) {
-                            // list the box with every brick it overlaps (one device atomic each, in parallel)
-                            const uint32_t di = bcast[2];
-                            const int bx0 = cw.x0 / (int)kBX, by0 = cw.y0 / (int)kBY, bz0 = cw.z0 / (int)kBZ;
-                            const int nbx = (cw.x0 + cw.ex - 1) / (int)kBX - bx0 + 1;
-                            const int nby = (cw.y0 + cw.ey - 1) / (int)kBY - by0 + 1;
-                            const int nbz = (cw.z0 + cw.ez - 1) / (int)kBZ - bz0 + 1;
-                            const uint32_t nbr = a.gw.nbx * a.gw.nby * a.gw.nbz;
-                            for (int t = threadIdx.x; t < nbx * nby * nbz; t += kW) {
-                                const uint32_t b = ((uint32_t)(bz0 + t / (nbx * nby)) * a.gw.nby +
-                                                    (uint32_t)(by0 + (t / nbx) % nby)) * a.gw.nbx +
-                                                   (uint32_t)(bx0 + t % nbx);
-                                const uint32_t at = atomicAdd(a.gw.brick_cnt + a.parity * nbr + b, 1u);
-                                if (at < kBrickList) a.gw.brick_list[((size_t)a.parity * nbr + b) * kBrickList + at] = di;
-                            }
-                        }
-                        int4* dst = a.gw.pool + (size_t)a.parity * a.gw.pool_cap + off;
-                        for (int c = threadIdx.x; c < cells; c += kW) {
-                            // sum the voxel's copies: (u64 vx<<32 | density), vy, vz
-                            uint64_t A = 0;
-                            uint32_t sy = 0, sz = 0;
-#pragma unroll
-                            for (int k = 0; k < kSlots; ++k) {
-                                const int4 raw = win[c * kSlots + k];
-                                A += ((uint64_t)(uint32_t)raw.y << 32) | (uint32_t)raw.x;
-                                sy += (uint32_t)raw.z;
-                                sz += (uint32_t)raw.w;
-                            }
-                            const int4 cell = make_int4((int)(uint32_t)(A >> 32), (int)sy, (int)sz, (int)(uint32_t)A);
-                            if (fits) {
-                                dst[c] = cell;
-                            } else if (cell.x | cell.y | cell.z | cell.w) {
-                                const int ax = c % cw.ex;
-                                const int rest = c / cw.ex;
-                                const size_t gi = (size_t)(cw.x0 + ax) + (size_t)(cw.y0 + rest % cw.ey) * G +
-                                                  (size_t)(cw.z0 + rest / cw.ey) * G * G;
-                                int* gcell = reinterpret_cast<int*>(a.gw.overflow + gi);
-                                atomicAdd(gcell + 0, cell.x);
-                                atomicAdd(gcell + 1, cell.y);
-                                atomicAdd(gcell + 2, cell.z);
-                                atomicAdd(gcell + 3, cell.w);
-                            }
-                        }
-                        __syncthreads();
                     }
+#pragma unroll
+                    for (uint32_t t = 0; t < kL; ++t) {
+                        if (!(t + 1 < cnt || (t + 1 == cnt && last_seg))) continue;
+                        const f3 q = grid_coord(X[t], origin, a.inv_h);
+                        const Rng rg = RP[t] ? unpack_block(RP[t]) : point_range(q, G);
+                        if (rng_empty(rg)) continue;
+                        if (rng_full(rg))
+                            splat_block(win, cw, rg, q, U[t], sl);
+                        else
+                            splat_any(win, cw, true, sl, a.gw.overflow, rg, q, U[t], G);
+                    }
+                }
+                __syncthreads();
+                dump_window(a, cw, cells, sl, (int)ws, win, bcast);
+                slot_used = true;
             }
             if (!slot_used && threadIdx.x == 0) {
                 // the slot's descriptor from two steps ago must not be read
@@ -1042,6 +1243,7 @@ __device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
                 cv = U[t];
             }
     }
+    if (SPLAT) deferred_chunks(a, wg, win, bcast, reinterpret_cast<uint32_t*>(red), own_k, n_own);
 }
 
 template <bool GATHER, bool SPLAT, int COL>
@@ -1146,13 +1348,18 @@ __device__ __forceinline__ void reduce_brick(const GridWork& gw, uint32_t G, uin
     const uint32_t* ctr = gw.counters + parity * 4;
     const bool ovf = ctr[2] != 0;
     const uint32_t nlisted = gw.brick_cnt[parity * nbr + brick];
-    if (lane == 0) {
-        gw.brick_cnt[(parity ^ 1u) * nbr + brick] = 0;  // for the step after next
-        if (reset_counters) {
-            uint32_t* nxt = gw.counters + (parity ^ 1u) * 4;
+    if (lane == 0) gw.brick_cnt[(parity ^ 1u) * nbr + brick] = 0;  // for the step after next
+    if (reset_counters) {
+        // the next step's counters, and the chunk claims of the previous
+        // step's deferred list (the next step uses the same half)
+        uint32_t* nxt = gw.counters + (parity ^ 1u) * 4;
+        const uint32_t nd = nxt[3];
+        for (uint32_t i = lane; i < nd && i < gw.nws; i += 64) gw.dclaim[(size_t)(parity ^ 1u) * gw.nws + i] = 0;
+        if (lane == 0) {
             nxt[0] = 0;
             nxt[1] = 0;
             nxt[2] = 0;
+            nxt[3] = 0;
         }
     }
     // nothing listed, no overflow, and zero already: the brick stays as it is

@@ -171,7 +171,7 @@ _SIGS = {
     "hs_write_strands": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_float)]),
     "hs_read_grid": (C.c_int, [_P, C.c_uint32, C.POINTER(HsGridCell)]),
     "hs_grid_near_limit": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint64)]),
-    "hs_grid_work": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint32 * 5)]),
+    "hs_grid_work": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint32 * 6)]),
     "hs_set_batch_graphs": (C.c_int, [_P, C.c_int]),
     "hs_batch_stats": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "hs_set_tile": (C.c_int, [_P, C.POINTER(HsTile)]),
@@ -641,10 +641,11 @@ class Groom:
         return out
 
     def grid_work(self):
-        """Last splatting step: dict of pool cells / capacity, boxes / capacity, fallback flag."""
-        v = (C.c_uint32 * 5)()
+        """Last splatting step: dict of pool cells / capacity, boxes / capacity, fallback flag,
+        deferred (multi-window) workgroup-segments."""
+        v = (C.c_uint32 * 6)()
         self.ctx._check(lib().hs_grid_work(self.ctx._h, self.id, C.byref(v)))
-        return dict(zip(["pool_cells", "pool_capacity", "boxes", "box_capacity", "fallback"], list(v)))
+        return dict(zip(["pool_cells", "pool_capacity", "boxes", "box_capacity", "fallback", "deferred"], list(v)))
 
     def grid_near_limit(self):
         """(cell, step) pairs so far with a GridCell component within 2x of the int32 limit."""
