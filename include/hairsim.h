@@ -256,6 +256,17 @@ int hs_export_strands(hs_ctx* ctx, uint32_t groom_id, float* strands_dev, uint32
  * float4 (w = 1), S*M*16 bytes of DEVICE memory `out_dev`, caller strand
  * order.  Enqueued on the library stream. */
 int hs_resample_strands(hs_ctx* ctx, uint32_t groom_id, uint32_t samples, float* out_dev);  /* 2 <= samples <= 2^20 */
+/* hs_read_strands for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc); no counterpart in the
+ * reference.  Same outputs, bit for bit, but the device staging copy lives in
+ * a fixed array of HS_NOALLOC_STAGING_BYTES per GPU that the library holds
+ * until the process ends, so no call allocates or frees anything.  A groom of
+ * more than HS_NOALLOC_STAGING_BYTES / 48 points (S*N) is refused with
+ * HS_ERR_INVALID.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (from several threads or contexts) race, so serialise
+ * them.  Blocks. */
+#define HS_NOALLOC_STAGING_BYTES (64u << 20)
+int hs_read_strands_noalloc(hs_ctx* ctx, uint32_t groom_id, float* strands_aos, hs_draw_indirect* indirect);
 /* Overwrite the groom's state from a reference-layout buffer (checkpoint
  * restore; clears any pending grid friction). */
 int hs_write_strands(hs_ctx* ctx, uint32_t groom_id, const float* strands_aos);

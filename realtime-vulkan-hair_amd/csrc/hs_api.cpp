@@ -587,7 +587,9 @@ int hs_create(hs_ctx** out, int device)
     hs_ctx* c = new (std::nothrow) hs_ctx();
     if (!c) return HS_ERR_OUT_OF_MEMORY;
     c->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
+    void* staging = nullptr;  // the fixed staging array of hs_read_strands_noalloc, looked up once per device
+    if (hipSetDevice(device) != hipSuccess || hsi::noalloc_scratch(device, &staging) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return HS_ERR_HIP;
     }

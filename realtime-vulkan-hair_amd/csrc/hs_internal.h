@@ -144,6 +144,9 @@ struct hs_ctx {
 };
 
 namespace hsi {
+// Device address of the fixed staging array of hs_read_strands_noalloc on
+// `device` (looked up once per device, which must be current).
+hipError_t noalloc_scratch(int device, void** out);
 // Adds a built groom to the context; returns its id.
 uint32_t ctx_adopt(hs_ctx* c, Groom&& g);
 // Removes a groom from the context without freeing its block.

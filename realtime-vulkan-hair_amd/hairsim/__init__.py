@@ -168,6 +168,7 @@ _SIGS = {
                           C.POINTER(HsStepParams)]),
     "hs_step_batch": (C.c_int, [_P, C.POINTER(HsBatchItem), C.c_uint32]),
     "hs_read_strands": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_float), C.POINTER(HsDrawIndirect)]),
+    "hs_read_strands_noalloc": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_float), C.POINTER(HsDrawIndirect)]),
     "hs_write_strands": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_float)]),
     "hs_read_grid": (C.c_int, [_P, C.c_uint32, C.POINTER(HsGridCell)]),
     "hs_grid_near_limit": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint64)]),

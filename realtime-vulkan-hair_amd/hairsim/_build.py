@@ -19,7 +19,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(ROOT, "csrc")
 OUT = os.path.join(PKG, "libhairsim.so")
 
-SOURCES = ["hs_kernels.hip", "hs_api.cpp", "hs_host.cpp", "hs_server.cpp", "hs_ingest.cpp"]
+SOURCES = ["hs_kernels.hip", "hs_noalloc.hip", "hs_api.cpp", "hs_host.cpp", "hs_server.cpp", "hs_ingest.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
