@@ -117,13 +117,15 @@ extern "C" {
 void hs_collider_from_trs(hs_collider* out, const float tr[3], const float rot_deg[3], const float sc[3])
 {
     if (!out || !tr || !rot_deg || !sc) return;
-    const double kDegToRad = 0.01745329251;  // DEG_TO_RAD, Scene.h:12
+    // the angle as Scene.h:30-32 forms it: rot * (float)DEG_TO_RAD in float
+    // (DEG_TO_RAD 0.01745329251, Scene.h:12); cos/sin and products in binary64
+    const float kDegToRad = (float)0.01745329251;
     const double t[3] = {tr[0], tr[1], tr[2]};
     const double s[3] = {sc[0], sc[1], sc[2]};
     M4 m = translate(t);
-    m = mul(m, rotate_axis(rot_deg[2] * kDegToRad, 2));
-    m = mul(m, rotate_axis(rot_deg[1] * kDegToRad, 1));
-    m = mul(m, rotate_axis(rot_deg[0] * kDegToRad, 0));
+    m = mul(m, rotate_axis((double)(rot_deg[2] * kDegToRad), 2));
+    m = mul(m, rotate_axis((double)(rot_deg[1] * kDegToRad), 1));
+    m = mul(m, rotate_axis((double)(rot_deg[0] * kDegToRad), 0));
     m = mul(m, scale(s));
     store(out, m);
 }
