@@ -377,9 +377,12 @@ int hs_initial_strands(const float* roots, const float* root_normals, uint32_t n
 /* Served simulation jobs (no reference counterpart: the reference steps one  */
 /* groom per frame, main.cpp:257-285).  A server owns a context, an HBM arena */
 /* of `hbm_budget` bytes and three threads: admission (uploads jobs in FIFO   */
-/* order while the budget and the per-step point limit allow), the step loop  */
-/* (one hs_step_batch over every admitted job per launch, so jobs that arrive */
-/* join the next launch) and completion (reads finished jobs back).           */
+/* order while the HBM budget allows), the step loop (each hs_step_batch      */
+/* takes resident jobs in round-robin order up to max_points_per_step and     */
+/* steps each once; it builds the next launch while the current one runs,    */
+/* with at most two launches queued) and completion (reads finished jobs      */
+/* back).  When the resident jobs hold more points than one launch, each     */
+/* job advances once every few launches.                                      */
 /* ------------------------------------------------------------------------ */
 
 /* Wind and collider-0 centre at time t (seconds since the job's start),
@@ -420,12 +423,17 @@ typedef struct hs_server_stats {
     uint64_t launches;            /* batched step launches                      */
     uint64_t job_steps;           /* sum over launches of the jobs stepped      */
     uint64_t point_updates;       /* sum over launches of S*N stepped           */
-    double p50_step_ms, p99_step_ms, max_step_ms;   /* per job-step, launch to  */
-                                                    /* completion on the device */
+    double p50_step_ms, p99_step_ms, max_step_ms;   /* per job-step: device     */
+                                                    /* time of its launch       */
     double p50_wait_ms, p99_wait_ms;                /* submit -> first step     */
     double busy_s;                /* time with at least one job admitted        */
     uint64_t peak_resident_bytes;
     uint32_t peak_jobs_per_launch;
+    double gpu_busy_s;            /* sum over launches of their device time;    */
+                                  /* gpu_busy_s / busy_s is the busy fraction   */
+    double p50_latency_ms, p99_latency_ms;  /* per job-step: host enqueue to    */
+                                  /* completion seen by the host, the wait      */
+                                  /* behind the launch queued before included   */
 } hs_server_stats;
 
 typedef struct hs_server hs_server;

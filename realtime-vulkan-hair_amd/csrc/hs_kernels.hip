@@ -1254,6 +1254,68 @@ __global__ __launch_bounds__(kW) void k_step(StepArgs a)
     step_body<GATHER, SPLAT, COL>(a, wg);
 }
 
+// Pointers a kernel reads out of device memory (the batched launches' argument
+// blob) are generic to the compiler, which then emits flat accesses: they wait
+// on the LDS counter as well as the memory counter, so every LDS wait in the
+// step body would also wait for outstanding HBM loads.  Every buffer here is
+// global memory: reading each pointer field as a global-address-space pointer
+// lets the compiler emit global loads, stores and atomics through it, as it
+// does for pointers taken from kernel arguments.
+template <class T>
+__device__ __forceinline__ T* global_ptr(T* const& field)
+{
+    using GT = __attribute__((address_space(1))) T*;
+    return (T*)*static_cast<const GT*>(static_cast<const void*>(&field));
+}
+
+__device__ __forceinline__ GridWork global_grid(const GridWork& in)
+{
+    GridWork w = in;
+    w.grid_i = global_ptr(in.grid_i);
+    w.grid_f = global_ptr(in.grid_f);
+    w.pool = global_ptr(in.pool);
+    w.desc = global_ptr(in.desc);
+    w.counters = global_ptr(in.counters);
+    w.overflow = global_ptr(in.overflow);
+    w.gbox = global_ptr(in.gbox);
+    w.brick_cnt = global_ptr(in.brick_cnt);
+    w.brick_list = global_ptr(in.brick_list);
+    w.brick_nz = global_ptr(in.brick_nz);
+    w.near_limit = global_ptr(in.near_limit);
+    w.dlist = global_ptr(in.dlist);
+    w.dclaim = global_ptr(in.dclaim);
+    w.dbox = global_ptr(in.dbox);
+    return w;
+}
+
+__device__ __forceinline__ MeshArgs global_mesh(const MeshArgs& in)
+{
+    MeshArgs m = in;
+    m.tris = global_ptr(in.tris);
+    m.spheres = global_ptr(in.spheres);
+    m.cell_start = global_ptr(in.cell_start);
+    m.cell_tris = global_ptr(in.cell_tris);
+    m.cell_spheres = global_ptr(in.cell_spheres);
+    m.cell_keys = global_ptr(in.cell_keys);
+    return m;
+}
+
+// A groom's arguments read from the blob, every pointer global.
+__device__ __forceinline__ StepArgs global_args(const StepArgs& in)
+{
+    StepArgs a = in;
+    a.pos = global_ptr(in.pos);
+    a.vel = global_ptr(in.vel);
+    a.corr = global_ptr(in.corr);
+    a.seg = global_ptr(in.seg);
+    a.noise = global_ptr(in.noise);
+    a.gw = global_grid(in.gw);
+    a.cols = global_ptr(in.cols);
+    a.mesh = global_mesh(in.mesh);
+    a.mesh_f = global_ptr(in.mesh_f);
+    return a;
+}
+
 // Many grooms in one launch: wg_start[g] is the first global workgroup of
 // groom g (prefix sum of their workgroup counts, ngroom+1 entries).
 __device__ __forceinline__ uint32_t find_groom(const uint32_t* __restrict__ start, uint32_t n, uint32_t x)
@@ -1278,8 +1340,7 @@ __global__ __launch_bounds__(kW) void k_step_batch(const StepArgs* __restrict__ 
     const uint32_t gwg = swz ? xcd_swizzle(blockIdx.x, total) : blockIdx.x;
     if (gwg >= total) return;
     const uint32_t g = find_groom(wg_start, ngroom, gwg);
-    // read in place (scalar loads from the argument blob where used)
-    const StepArgs& a = args[g];
+    const StepArgs a = global_args(args[g]);
     step_body<GATHER, SPLAT, COL>(a, gwg - wg_start[g]);
 }
 
@@ -1295,7 +1356,8 @@ __global__ __launch_bounds__(256) void k_mesh_force_batch(const StepArgs* __rest
     const StepArgs& a = args[idx[m]];
     const uint32_t plane = a.N * a.Sp;
     const uint32_t i = (blockIdx.x - blk_start[m]) * 256u + threadIdx.x;
-    if (i < plane) mesh_point(a.mesh, a.pos, plane, a.k, const_cast<float4*>(a.mesh_f), i);
+    if (i < plane)
+        mesh_point(global_mesh(a.mesh), global_ptr(a.pos), plane, a.k, const_cast<float4*>(global_ptr(a.mesh_f)), i);
 }
 
 // ---------------------------------------------------------------------------
@@ -1451,7 +1513,7 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce_batch(const StepArgs*
     if (gb >= total) return;
     const uint32_t g = __builtin_amdgcn_readfirstlane(find_groom(brick_start, ngroom, gb));
     const uint32_t b = gb - brick_start[g];
-    reduce_brick(args[g].gw, args[g].G, args[g].parity, b, b == 0);
+    reduce_brick(global_grid(args[g].gw), args[g].G, args[g].parity, b, b == 0);
 }
 
 // ---------------------------------------------------------------------------

@@ -3,16 +3,21 @@
 // The reference steps a single groom once per rendered frame from one host
 // thread (src/main.cpp:257-285, src/Renderer.cpp:2309-2319); it has no job
 // model.  This file adds one on top of the same step: jobs (a groom, a step
-// count and a keyframed wind/sphere trajectory) are admitted while an HBM
-// budget and a per-launch point limit allow, stay resident in one device
-// arena, and every admitted job advances by one step in each batched launch,
-// so jobs that arrive while others run join the next launch (continuous
-// batching).  Each job's result equals stepping it alone with hs_step: the
-// batched kernels compute every groom exactly as the single-groom ones do.
+// count and a keyframed wind/sphere trajectory) are admitted while the HBM
+// budget allows and stay resident in one device arena.  Each batched launch
+// takes resident jobs in round-robin order up to the per-launch point limit
+// and advances each of them by one step, so jobs that arrive while others
+// run join a following launch (continuous batching).  Each job's result
+// equals stepping it alone with hs_step: the batched kernels compute every
+// groom exactly as the single-groom ones do, and a job's steps run in order
+// on one stream.
 //
 // Threads: admission (FIFO; uploads on its own stream), the step loop (owns
 // the context and its stream) and completion (reads finished jobs back on a
-// third stream while the next launch runs).
+// third stream while later launches run).  The step loop keeps up to
+// kLaunchDepth launches queued on the stream: it builds launch k+1 on the
+// host while launch k runs and only then waits for launch k-1, so the device
+// does not idle between launches.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -144,30 +149,41 @@ double weighted_percentile(std::vector<std::pair<double, uint64_t>> v, double q)
     return v.back().first;
 }
 
+// One batched launch on the stream: timing events around it, when the host
+// enqueued it, what it stepped and the jobs whose last step it holds.
+struct Flight {
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    Clock::time_point enq;
+    uint32_t jobs = 0;
+    uint64_t points = 0;
+    std::vector<Job*> last;
+};
+
 }  // namespace
 
 struct hs_server {
+    static constexpr int kLaunchDepth = 2;  // launches queued on the stream at once
     hs_server_config cfg{};
     hs_ctx* ctx = nullptr;
     Arena arena;
     hipStream_t up = nullptr, down = nullptr;
-    hipEvent_t step_done = nullptr;
+    Flight flight[kLaunchDepth];
 
     std::mutex m;
     std::condition_variable cv_admit, cv_loop, cv_read, cv_done;
     std::deque<Job*> queue;     // submitted, not admitted
-    std::vector<Job*> ready;    // uploaded, joins the next launch
+    std::vector<Job*> ready;    // uploaded, joins a following launch
     std::deque<Job*> finished;  // last step done, to be read back
     std::unordered_map<uint64_t, std::unique_ptr<Job>> jobs;
     uint64_t next_id = 1;
     uint64_t resident = 0;       // bytes of admitted jobs (arena)
-    uint64_t admitted_points = 0;
     uint64_t in_system = 0;      // admitted and not yet done
     bool stop_admit = false, stop_loop = false, stop_read = false;
     std::thread t_admit, t_loop, t_read;
 
     hs_server_stats st{};
-    std::vector<std::pair<double, uint64_t>> step_ms;  // (launch ms, jobs in it)
+    std::vector<std::pair<double, uint64_t>> step_ms;     // (launch device ms, jobs in it)
+    std::vector<std::pair<double, uint64_t>> latency_ms;  // (enqueue -> completion ms, jobs in it)
     std::vector<std::pair<double, uint64_t>> wait_ms;
     int err_code = 0;
     std::string err_msg;
@@ -179,15 +195,27 @@ struct hs_server {
             err_msg = msg;
         }
     }
-    bool fits(const Job* j) const
-    {
-        return resident + j->bytes + j->upload <= cfg.hbm_budget &&
-               admitted_points + j->points <= cfg.max_points_per_step;
-    }
+    bool fits(const Job* j) const { return resident + j->bytes + j->upload <= cfg.hbm_budget; }
+    void fail_job(Job* j);
     void admit_loop();
     void step_loop();
     void read_loop();
 };
+
+// A resident job whose device state is unknown after a failed launch: its
+// memory goes back to the arena (the caller has drained the stream) and
+// waiters see the error.  Called with `m` held.
+void hs_server::fail_job(Job* j)
+{
+    hsi::Groom g;
+    if (hsi::ctx_take(ctx, j->groom_id, &g)) hsi::groom_release(g);
+    hsi::groom_release(j->g);  // a job whose last step was enqueued holds its groom itself
+    if (j->staging) arena.release(j->staging);
+    j->staging = nullptr;
+    resident -= j->bytes;
+    in_system -= 1;
+    j->state = JobState::Failed;
+}
 
 void hs_server::admit_loop()
 {
@@ -199,7 +227,6 @@ void hs_server::admit_loop()
         Job* j = queue.front();
         queue.pop_front();
         resident += j->bytes;
-        admitted_points += j->points;
         in_system += 1;
         st.admitted += 1;
         st.peak_resident_bytes = std::max(st.peak_resident_bytes, resident);
@@ -217,7 +244,6 @@ void hs_server::admit_loop()
         if (rc == HS_ERR_OUT_OF_MEMORY || e == hipErrorOutOfMemory) {
             // the arena is fragmented: wait for a release unless nothing else holds memory
             resident -= j->bytes;
-            admitted_points -= j->points;
             in_system -= 1;
             st.admitted -= 1;
             if (resident == 0) {
@@ -233,7 +259,6 @@ void hs_server::admit_loop()
         }
         if (rc != HS_OK || e != hipSuccess) {
             resident -= j->bytes;
-            admitted_points -= j->points;
             in_system -= 1;
             j->state = JobState::Failed;
             set_error(rc != HS_OK ? rc : HS_ERR_HIP, "hs_server admission: " + why);
@@ -248,16 +273,68 @@ void hs_server::admit_loop()
 void hs_server::step_loop()
 {
     (void)hipSetDevice(cfg.device);
-    std::vector<Job*> active;
+    std::deque<Job*> active;  // resident jobs in round-robin order
+    std::vector<Job*> batch, skipped;
     std::vector<hs_batch_item> items;
     std::vector<hs_collider> cols;
     std::vector<hs_step_params> params;
+    int head = 0, inflight = 0;
     Clock::time_point busy_since{};
     bool busy = false;
+
+    // After a failed enqueue or launch: drain the stream, then fail every
+    // resident job (those stepped in the failed launch included).
+    auto fail_all = [&](int code, const std::string& msg) {
+        (void)hipStreamSynchronize(ctx->stream);
+        std::lock_guard<std::mutex> lk(m);
+        set_error(code, msg);
+        for (int q = 0; q < inflight; ++q) {
+            Flight& f = flight[(head + q) % kLaunchDepth];
+            for (Job* j : f.last) fail_job(j);
+            f.last.clear();
+        }
+        inflight = 0;
+        for (Job* j : batch) fail_job(j);
+        for (Job* j : active) fail_job(j);
+        batch.clear();
+        active.clear();
+        cv_done.notify_all();
+        cv_admit.notify_all();
+    };
+    // Wait for the oldest launch on the stream, record its times and hand
+    // the jobs it finished to the read-back thread.
+    auto retire = [&]() -> bool {
+        Flight& f = flight[head];
+        hipError_t e = hipEventSynchronize(f.t1);
+        const Clock::time_point seen = Clock::now();
+        float dev_ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&dev_ms, f.t0, f.t1);
+        if (e != hipSuccess) {
+            fail_all(HS_ERR_HIP, std::string("hs_server step: ") + hipGetErrorString(e));
+            return false;
+        }
+        head = (head + 1) % kLaunchDepth;
+        --inflight;
+        std::lock_guard<std::mutex> lk(m);
+        st.launches += 1;
+        st.job_steps += f.jobs;
+        st.point_updates += f.points;
+        st.gpu_busy_s += dev_ms * 1e-3;
+        step_ms.emplace_back(dev_ms, f.jobs);
+        latency_ms.emplace_back(std::chrono::duration<double, std::milli>(seen - f.enq).count(), f.jobs);
+        for (Job* j : f.last) {
+            j->state = JobState::Reading;
+            finished.push_back(j);
+        }
+        if (!f.last.empty()) cv_read.notify_one();
+        f.last.clear();
+        return true;
+    };
+
     while (true) {
         {
             std::unique_lock<std::mutex> lk(m);
-            if (active.empty()) {
+            if (active.empty() && inflight == 0) {
                 if (busy) {
                     st.busy_s += seconds_since(busy_since);
                     busy = false;
@@ -276,13 +353,34 @@ void hs_server::step_loop()
             busy = true;
             busy_since = Clock::now();
         }
-        const size_t n = active.size();
+        if (active.empty()) {
+            retire();  // only launches on the stream are left
+            continue;
+        }
+        // This launch: jobs from the front of the round-robin order while
+        // they fit the point limit.  A job that does not fit keeps its place
+        // at the front, so it leads a following launch (every job fits one
+        // launch alone: submit checks that).
+        batch.clear();
+        skipped.clear();
+        uint64_t pts = 0;
+        for (size_t scanned = 0; !active.empty() && scanned < 4096 && pts < cfg.max_points_per_step; ++scanned) {
+            Job* j = active.front();
+            active.pop_front();
+            if (pts + j->points <= cfg.max_points_per_step) {
+                batch.push_back(j);
+                pts += j->points;
+            } else {
+                skipped.push_back(j);
+            }
+        }
+        for (auto it = skipped.rbegin(); it != skipped.rend(); ++it) active.push_front(*it);
+        const size_t n = batch.size();
         items.resize(n);
         cols.resize(n);
         params.resize(n);
-        uint64_t pts = 0;
         for (size_t k = 0; k < n; ++k) {
-            Job* j = active[k];
+            Job* j = batch[k];
             const float t = (float)(j->done_steps + 1) * j->spec.dt;
             float wind[3], sph[3];
             hs_keyframes_eval(j->spec.keys, j->spec.num_keys, t, wind, sph);
@@ -295,63 +393,41 @@ void hs_server::step_loop()
             items[k].colliders = &cols[k];
             items[k].num_colliders = params[k].num_colliders > 0 ? 1u : 0u;
             items[k].params = &params[k];
-            pts += j->points;
         }
-        const Clock::time_point t0 = Clock::now();
-        int rc = hs_step_batch(ctx, items.data(), (uint32_t)n);
-        hipError_t e = hipSuccess;
-        if (rc == HS_OK) {
-            e = hipEventRecord(step_done, ctx->stream);
-            if (e == hipSuccess) e = hipEventSynchronize(step_done);
-        }
-        const double ms = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
-        std::lock_guard<std::mutex> lk(m);
+        // the host work above overlapped the launches on the stream; now
+        // make room for this one
+        if (inflight == kLaunchDepth && !retire()) continue;
+        Flight& f = flight[(head + inflight) % kLaunchDepth];
+        f.enq = Clock::now();
+        int rc = HS_OK;
+        hipError_t e = hipEventRecord(f.t0, ctx->stream);
+        if (e == hipSuccess) rc = hs_step_batch(ctx, items.data(), (uint32_t)n);
+        if (e == hipSuccess && rc == HS_OK) e = hipEventRecord(f.t1, ctx->stream);
         if (rc != HS_OK || e != hipSuccess) {
-            set_error(rc != HS_OK ? rc : HS_ERR_HIP,
-                      rc != HS_OK ? std::string("hs_server step: ") + ctx->err
-                                  : std::string("hs_server step: ") + hipGetErrorString(e));
-            // the device state of these jobs is unknown: fail them
-            for (Job* j : active) {
-                hsi::Groom g;
-                if (hsi::ctx_take(ctx, j->groom_id, &g)) hsi::groom_release(g);
-                if (j->staging) arena.release(j->staging);
-                resident -= j->bytes;
-                admitted_points -= j->points;
-                in_system -= 1;
-                j->state = JobState::Failed;
-            }
-            active.clear();
-            cv_done.notify_all();
-            cv_admit.notify_all();
+            // the device state of the resident jobs is unknown: fail them
+            fail_all(rc != HS_OK ? rc : HS_ERR_HIP, rc != HS_OK ? std::string("hs_server step: ") + ctx->err
+                                                                  : std::string("hs_server step: ") +
+                                                                        hipGetErrorString(e));
             continue;
         }
-        st.launches += 1;
-        st.job_steps += n;
-        st.point_updates += pts;
+        ++inflight;
+        f.jobs = (uint32_t)n;
+        f.points = pts;
+        std::lock_guard<std::mutex> lk(m);
         st.peak_jobs_per_launch = std::max<uint32_t>(st.peak_jobs_per_launch, (uint32_t)n);
-        step_ms.emplace_back(ms, n);
-        size_t keep = 0;
-        bool freed = false;
-        for (size_t k = 0; k < n; ++k) {
-            Job* j = active[k];
+        for (Job* j : batch) {
             if (j->done_steps == 0)
-                wait_ms.emplace_back(std::chrono::duration<double, std::milli>(t0 - j->t_submit).count(), 1);
+                wait_ms.emplace_back(std::chrono::duration<double, std::milli>(f.enq - j->t_submit).count(), 1);
             j->done_steps += 1;
             if (j->done_steps >= j->spec.steps) {
+                // out of the context now; its memory stays until read back
                 hsi::ctx_take(ctx, j->groom_id, &j->g);
-                j->state = JobState::Reading;
-                admitted_points -= j->points;
-                finished.push_back(j);
-                freed = true;
+                f.last.push_back(j);
             } else {
-                active[keep++] = j;
+                active.push_back(j);
             }
         }
-        active.resize(keep);
-        if (freed) {
-            cv_read.notify_one();
-            cv_admit.notify_all();
-        }
+        batch.clear();
     }
 }
 
@@ -365,7 +441,7 @@ void hs_server::read_loop()
         Job* j = finished.front();
         finished.pop_front();
         lk.unlock();
-        // the step loop waited for the last launch: the state is final
+        // the step loop saw the job's last launch complete: the state is final
         hipError_t e = hipSuccess;
         if (j->spec.result_aos) {
             e = hsi::groom_unpack(j->g, static_cast<float*>(j->staging), down);
@@ -438,11 +514,17 @@ int hs_server_create(const hs_server_config* cfg, hs_server** out)
     hipError_t e = s->arena.init(cfg->hbm_budget);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->up, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->down, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->step_done, hipEventDisableTiming);
+    for (Flight& f : s->flight) {
+        if (e == hipSuccess) e = hipEventCreate(&f.t0);
+        if (e == hipSuccess) e = hipEventCreate(&f.t1);
+    }
     if (e != hipSuccess) {
         if (s->up) (void)hipStreamDestroy(s->up);
         if (s->down) (void)hipStreamDestroy(s->down);
-        if (s->step_done) (void)hipEventDestroy(s->step_done);
+        for (Flight& f : s->flight) {
+            if (f.t0) (void)hipEventDestroy(f.t0);
+            if (f.t1) (void)hipEventDestroy(f.t1);
+        }
         hs_destroy(s->ctx);
         return e == hipErrorOutOfMemory ? HS_ERR_OUT_OF_MEMORY : HS_ERR_HIP;
     }
@@ -536,6 +618,8 @@ int hs_server_get_stats(hs_server* s, hs_server_stats* out)
     double mx = 0.0;
     for (auto& e : s->step_ms) mx = std::max(mx, e.first);
     out->max_step_ms = mx;
+    out->p50_latency_ms = weighted_percentile(s->latency_ms, 0.50);
+    out->p99_latency_ms = weighted_percentile(s->latency_ms, 0.99);
     out->p50_wait_ms = weighted_percentile(s->wait_ms, 0.50);
     out->p99_wait_ms = weighted_percentile(s->wait_ms, 0.99);
     return HS_OK;
@@ -576,7 +660,10 @@ void hs_server_destroy(hs_server* s)
     hs_destroy(s->ctx);
     (void)hipStreamDestroy(s->up);
     (void)hipStreamDestroy(s->down);
-    (void)hipEventDestroy(s->step_done);
+    for (Flight& f : s->flight) {
+        (void)hipEventDestroy(f.t0);
+        (void)hipEventDestroy(f.t1);
+    }
     delete s;
 }
 

@@ -147,6 +147,7 @@ class HsServerStats(C.Structure):
         ("p50_step_ms", C.c_double), ("p99_step_ms", C.c_double), ("max_step_ms", C.c_double),
         ("p50_wait_ms", C.c_double), ("p99_wait_ms", C.c_double), ("busy_s", C.c_double),
         ("peak_resident_bytes", C.c_uint64), ("peak_jobs_per_launch", C.c_uint32),
+        ("gpu_busy_s", C.c_double), ("p50_latency_ms", C.c_double), ("p99_latency_ms", C.c_double),
     ]
 
 

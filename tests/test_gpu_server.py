@@ -85,6 +85,32 @@ def test_served_jobs_equal_alone(ctx):
     assert 1 < st["peak_jobs_per_launch"] < len(jobs)
     assert st["job_steps"] == sum(j["steps"] for j in jobs)
     assert st["p99_step_ms"] >= st["p50_step_ms"] > 0.0
+    assert st["p99_latency_ms"] >= st["p50_latency_ms"] >= st["p50_step_ms"] * 0.5
+    assert 0.0 < st["gpu_busy_s"] <= st["busy_s"] * 1.05
+
+
+def test_round_robin_launches_equal_alone(ctx):
+    """Every job resident at once (a large HBM budget) but a point limit of
+    about three jobs per launch: launches take jobs in round-robin order, two
+    launches are queued at a time, and every job still ends bit-identical to
+    stepping it alone."""
+    jobs = _job_set(91, 8)
+    refs = [_alone(ctx, j, forcing=i % 3) for i, j in enumerate(jobs)]
+    pts = [j["aos"].size // 12 for j in jobs]
+    cap = int(sum(sorted(pts)[-3:]))
+    with hairsim.Server(0, hbm_budget=1 << 30, max_points_per_step=cap) as srv:
+        ids = [srv.submit(j["aos"], j["N"], j["steps"], DT, j["keys"], _params(forcing=i % 3), seg_length=j["seg"],
+                          wind_noise=j["noise"], **GRID)[0] for i, j in enumerate(jobs)]
+        for i, jid in enumerate(ids):
+            assert_bitwise(srv.wait(jid, timeout=60.0), refs[i], f"job {i}", jobs[i]["N"])
+        srv.drain()
+        st = srv.stats()
+    assert st["completed"] == len(jobs)
+    assert st["job_steps"] == sum(j["steps"] for j in jobs)
+    assert st["point_updates"] == sum(p * j["steps"] for p, j in zip(pts, jobs))
+    # launches never exceeded the point limit, so they took turns
+    assert st["point_updates"] <= st["launches"] * cap
+    assert st["launches"] > max(j["steps"] for j in jobs)
 
 
 def test_server_rejects_and_cancels(ctx):
@@ -99,9 +125,11 @@ def test_server_rejects_and_cancels(ctx):
         with pytest.raises(hairsim.HairSimError) as e:
             srv.submit(j["aos"], j["N"], 3, DT, j["keys"], _params(), seg_length=j["seg"], **GRID)
         assert e.value.code == hairsim.HS_ERR_BUDGET
-    # a one-job-at-a-time server: queued jobs can be cancelled; the admitted one finishes
-    pts = max(x["aos"].size // 12 for x in jobs)
-    with hairsim.Server(0, hbm_budget=1 << 30, max_points_per_step=pts) as srv:
+    # a one-job-at-a-time server (the budget holds one job): queued jobs can
+    # be cancelled; the admitted one finishes
+    one = max(hairsim.groom_bytes(x["aos"].size // (12 * x["N"]), x["N"], 16) + 2 * x["aos"].nbytes + 512
+              for x in jobs)
+    with hairsim.Server(0, hbm_budget=one, max_points_per_step=1 << 24) as srv:
         ids = [srv.submit(x["aos"], x["N"], 40, DT, x["keys"], _params(), seg_length=x["seg"], **GRID)[0]
                for x in jobs]
         n = srv.cancel_pending()
