@@ -167,6 +167,30 @@ def run_scene(ctx, hairsim, scenes, wl, mesh_id, warmup, steps, profile):
     return el, kern, out, near
 
 
+def tune_tile(ctx, hairsim, scenes, wl, mesh_id, warmup):
+    """The tile (shape, LDS windows, workgroup order) for the timed steps:
+    hs_autotune_batch on copies of a groom stepped through the untimed steps,
+    each candidate timed over the first timed steps' inputs."""
+    g = wl.grooms[0]
+    groom = ctx.groom(g.strands, g.N, seg_length=g.seg, wind_noise=g.noise, noise_dir=g.noise_dir,
+                      grid_dim=g.grid_dim, grid_origin=g.grid_origin, grid_h=g.grid_h)
+    for k in range(warmup):
+        groom.step(wl.dt, float(wl.time(k)), [scenes.sphere_collider(wl.sphere(k))], make_params(hairsim, wl, k, mesh_id))
+    b = ctx.batch([groom])
+    b._params[0] = make_params(hairsim, wl, warmup, mesh_id)
+    b.set_colliders(scenes.sphere_collider(wl.sphere(warmup))[None, None])
+    b.set_time(wl.dt, float(wl.time(warmup)))
+    tile, ms = ctx.autotune(b, steps=3)
+    ctx.sync()
+    groom.destroy()
+    return tile, ms
+
+
+def tile_record(tile, tuned_ms):
+    return {"gather_cells": tile[0], "splat_cells": tile[1], "xcd_order": tile[2], "strands_per_workgroup": tile[3],
+            "points_per_segment": tile[4], "strands_per_wavefront": 64, "autotuned_ms_per_step": tuned_ms}
+
+
 def bench_single(args, hairsim, scenes, wl):
     g = wl.grooms[0]
     warmup, steps = scene_steps(args, wl)
@@ -175,6 +199,8 @@ def bench_single(args, hairsim, scenes, wl):
     mesh_id = 0
     if wl.mesh is not None:
         mesh_id = ctx.mesh(*wl.mesh, grid_dim=wl.mesh_grid, margin=wl.mesh_margin)
+    tile, tuned_ms = tune_tile(ctx, hairsim, scenes, wl, mesh_id, warmup) if not args.no_autotune \
+        else (ctx.get_tile(), None)
     # the timed run, then the same scene steps again from a fresh copy of the
     # groom with kernel events: the kernel averages describe the timed steps
     el, _, out, near = run_scene(ctx, hairsim, scenes, wl, mesh_id, warmup, steps, profile=False)
@@ -234,6 +260,7 @@ def bench_single(args, hairsim, scenes, wl):
                      "reduce_kernel_avg_us": reduce_us, "reduce_algorithmic_bytes": b_grid,
                      "whole_step_achieved": whole, "whole_step_frac": whole / PEAK_HBM_GBS},
         "grid_cells_near_int32_limit": near,
+        "tile": tile_record(tile, tuned_ms),
     }
     if b_mesh:
         mesh_gbs = b_mesh / (mesh_us * 1e-6) / 1e9
@@ -281,11 +308,12 @@ def bench_batched(args, hairsim, scenes, wl):
         b.step()
 
     args.warmup, args.steps = scene_steps(args, wl)
-    # tile choice: timed on copies of the grooms (their state is untouched)
-    b.set_time(wl.dt, float(wl.time(0)))
-    tile, tuned_ms = ctx.autotune(b, steps=3) if not args.no_autotune else (ctx.get_tile(), None)
     for k in range(args.warmup):
         batch_step(k)
+    # tile choice: timed on copies of the grooms at the first timed step
+    # (their state is untouched)
+    b.set_time(wl.dt, float(wl.time(args.warmup)))
+    tile, tuned_ms = ctx.autotune(b, steps=3) if not args.no_autotune else (ctx.get_tile(), None)
     ctx.sync()
     t0 = time.perf_counter()
     for k in range(args.warmup, args.warmup + args.steps):
@@ -371,8 +399,7 @@ def bench_batched(args, hairsim, scenes, wl):
                      "whole_step_achieved": whole, "whole_step_frac": whole / PEAK_HBM_GBS},
         "per_groom_launch": {"value": plain, "steps": plain_steps, "ms_per_step": el_plain / plain_steps * 1e3,
                              "speedup_of_batched": value / plain},
-        "tile": {"gather_cells": tile[0], "splat_cells": tile[1], "xcd_order": tile[2],
-                 "autotuned_ms_per_step": tuned_ms},
+        "tile": tile_record(tile, tuned_ms),
     }
     print(json.dumps(res), flush=True)
 
@@ -561,7 +588,7 @@ def main():
     ap.add_argument("--max-points", type=int, default=12_000_000, help="served: points per batched launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="batched: eager launches instead of graphs")
-    ap.add_argument("--no-autotune", action="store_true", help="batched: keep the default tile")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default tile")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="run only a plain elementwise kernel over MIB MiB (byte-counter calibration)")
     args = ap.parse_args()

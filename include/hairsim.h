@@ -219,18 +219,27 @@ int hs_set_batch_graphs(hs_ctx* ctx, int on);
 /* hs_step_batch calls so far and the graphs captured for them. */
 int hs_batch_stats(const hs_ctx* ctx, uint64_t* batch_calls, uint64_t* graph_captures);
 
-/* Tile choices of the step kernels (no reference counterpart).  Results are
- * identical for every choice; only the speed changes. */
+/* Tile choices of the step kernels (no reference counterpart; the reference
+ * fixes one strand per invocation in workgroups of 32, compute.comp:4,13, and
+ * Renderer.cpp:2070).  Results are identical for every choice; only the speed
+ * changes.  One lane steps one strand (64 strands per wavefront); the
+ * workgroup takes `strands` strands and walks them root to tip `points`
+ * points at a time.  (strands, points) is one of the built shapes: (128, 16),
+ * (256, 8), (256, 12).  A change of shape takes effect on the next step. */
 typedef struct hs_tile {
     uint32_t gather_cells;  /* voxels of the LDS window the friction gather stages, 64..2048 */
     uint32_t splat_cells;   /* voxels of one LDS splat window, 64..2048                   */
     uint32_t xcd_order;     /* 1: each XCD takes a contiguous run of workgroups           */
+    uint32_t strands;       /* strands per workgroup                                      */
+    uint32_t points;        /* points per segment of the root-to-tip walk                 */
 } hs_tile;
 int hs_set_tile(hs_ctx* ctx, const hs_tile* tile);
 int hs_get_tile(const hs_ctx* ctx, hs_tile* tile);
-/* Times every candidate tile on copies of the batch's grooms (their state is
- * left untouched), `steps` batched steps each after one untimed step, and
- * keeps the fastest for later steps.  best / best_ms may be NULL. */
+/* Times every candidate tile (each built shape x gather and splat window sizes
+ * x workgroup order) on copies of the batch's grooms (their state is left
+ * untouched): each candidate starts from the grooms' present state and runs
+ * one untimed and `steps` timed batched steps.  Keeps the fastest for later
+ * steps (hs_step too).  best / best_ms may be NULL. */
 int hs_autotune_batch(hs_ctx* ctx, const hs_batch_item* items, uint32_t num_items, uint32_t steps,
                       hs_tile* best, double* best_ms);
 

@@ -28,20 +28,28 @@ using namespace hsi;
 
 namespace hsi {
 
+ShapeDims shape_dims(int shape, uint32_t S, uint32_t N)
+{
+    const hs::TileShape t = hs::kShapes[shape];
+    return ShapeDims{(S + t.strands - 1) / t.strands, (N - 1 + t.points - 1) / t.points};
+}
+
 Layout layout_of(uint32_t S, uint32_t N, uint32_t G)
 {
     Layout l;
-    l.Sp = (uint32_t)round_up(S, hs::kW);
-    l.nwg = l.Sp / hs::kW;
-    l.nseg = (N - 1 + hs::kL - 1) / hs::kL;
+    l.Sp = (uint32_t)round_up(S, hs::kMaxTileStrands);
+    l.nws_cap = 0;
+    for (int k = 0; k < hs::kNumShapes; ++k) {
+        const ShapeDims d = shape_dims(k, S, N);
+        l.nws_cap = std::max(l.nws_cap, d.nwg * d.nseg);
+    }
     l.pool_cap = 0;
     l.desc_cap = 0;
     if (G) {
-        // partial boxes: room for half a cell per point per step, at least 64k cells
         // each workgroup-segment's slot (its first box always fits), then a
         // shared part for further chunks: half a cell per point per step (at
         // least 64k cells) and three boxes per segment
-        const uint64_t nws = (uint64_t)l.nwg * l.nseg;
+        const uint64_t nws = l.nws_cap;
         l.pool_cap = nws * hs::kSlotCells + std::max<uint64_t>(65536, (uint64_t)S * (N - 1) / 2);
         l.desc_cap = nws + 3 * nws + 4096;
     }
@@ -80,17 +88,18 @@ uint64_t carve(Groom& g, uint8_t* base)
         take(g.gw.pool, 2 * L.pool_cap * 16);
         take(g.gw.desc, 2 * L.desc_cap * hs::kDescInts * 4);
         take(g.gw.counters, 8 * 4);
-        take(g.gw.gbox, (uint64_t)L.nwg * L.nseg * 6 * 4);
+        take(g.gw.gbox, (uint64_t)L.nws_cap * 6 * 4);
         take(g.gw.brick_cnt, bricks(g.G) * 2 * 4);
         take(g.gw.brick_list, bricks(g.G) * 2 * hs::kBrickList * 4);
         take(g.gw.brick_nz, bricks(g.G) * 4);  // zero: every brick of the zeroed grid is empty
         take(g.gw.near_limit, 8);
-        take(g.gw.dlist, 2ull * L.nwg * L.nseg * 8);
-        take(g.gw.dclaim, 2ull * L.nwg * L.nseg * 4);
-        take(g.gw.dbox, (uint64_t)L.nwg * L.nseg * hs::kDBoxInts * 4);
+        take(g.gw.dlist, 2ull * L.nws_cap * 8);
+        take(g.gw.dclaim, 2ull * L.nws_cap * 4);
+        take(g.gw.dbox, (uint64_t)L.nws_cap * hs::kDBoxInts * 4);
         g.gw.pool_cap = (uint32_t)L.pool_cap;
         g.gw.desc_cap = (uint32_t)L.desc_cap;
-        g.gw.nws = L.nwg * L.nseg;
+        g.gw.nws = L.nws_cap;  // each launch sets its tile shape's count
+        g.gw.nws_cap = L.nws_cap;
         g.gw.nbx = bricks_x(g.G);
         g.gw.nby = bricks_y(g.G);
         g.gw.nbz = bricks_z(g.G);
@@ -244,10 +253,12 @@ float sqrt_lower_threshold(float r)
 }
 
 // Physical strand order: a balanced k-d split of the roots into leaves of
-// exactly kW strands (one workgroup each; only the last leaf may be short).
-// Every workgroup's strands start close together, so the voxels they touch
-// form a compact box; unlike a space-filling curve there is no jump between
-// distant patches inside one workgroup.
+// exactly kLeafStrands strands (only the last leaf may be short).  Nodes
+// larger than kMaxTileStrands split at a multiple of kMaxTileStrands, so every
+// aligned run of kMaxTileStrands slots is one subtree: a workgroup of any tile
+// shape (whole leaves, aligned) holds strands that start close together, and
+// the voxels they touch form a compact box; unlike a space-filling curve
+// there is no jump between distant patches inside one workgroup.
 std::vector<uint32_t> kd_perm(const float* aos, uint32_t S, uint32_t N)
 {
     std::vector<float> root(3ull * S);
@@ -266,7 +277,7 @@ std::vector<uint32_t> kd_perm(const float* aos, uint32_t S, uint32_t N)
         const Range r = stack.back();
         stack.pop_back();
         const uint32_t n = r.e - r.b;
-        if (n <= hs::kW) continue;
+        if (n <= hs::kLeafStrands) continue;
         float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
         for (uint32_t i = r.b; i < r.e; ++i)
             for (int k = 0; k < 3; ++k) {
@@ -276,10 +287,12 @@ std::vector<uint32_t> kd_perm(const float* aos, uint32_t S, uint32_t N)
         int ax = 0;
         for (int k = 1; k < 3; ++k)
             if (hi[k] - lo[k] > hi[ax] - lo[ax]) ax = k;
-        // split at the multiple of kW nearest the median: the left part is
-        // always whole workgroups, the remainder travels right
-        uint32_t mid = (uint32_t)(((n / 2) + hs::kW / 2) / hs::kW) * hs::kW;
-        mid = std::min(std::max(mid, (uint32_t)hs::kW), n - 1);
+        // split at the multiple of q nearest the median: the left part is
+        // always whole leaves (whole largest workgroups above that size), the
+        // remainder travels right
+        const uint32_t q = n > hs::kMaxTileStrands ? hs::kMaxTileStrands : hs::kLeafStrands;
+        uint32_t mid = (uint32_t)(((n / 2) + q / 2) / q) * q;
+        mid = std::min(std::max(mid, q), n - 1);
         auto cmp = [&](uint32_t a, uint32_t b) {
             const float fa = root[3ull * a + ax], fb = root[3ull * b + ax];
             return fa < fb || (fa == fb && a < b);
@@ -323,9 +336,12 @@ int build_args(hs_ctx* c, Groom& g, const hs_time* t, const hs_collider* cols, u
     a.S = g.S;
     a.N = g.N;
     a.Sp = g.L.Sp;
-    a.nwg = g.L.nwg;
-    a.nseg = g.L.nseg;
+    const ShapeDims sd = shape_dims(c->shape, g.S, g.N);
+    a.nwg = sd.nwg;
+    a.nseg = sd.nseg;
     a.gw = g.gw;
+    a.gw.nws = sd.nwg * sd.nseg;
+    a.gbox_ok = g.last_shape == c->shape ? 1u : 0u;
     a.G = g.G;
     a.parity = g.parity;
     a.tag = g.steps % 0x0fffffffu + 1u;  // 28 bits, never 0: the deferred list starts zero-filled
@@ -389,11 +405,13 @@ int build_args(hs_ctx* c, Groom& g, const hs_time* t, const hs_collider* cols, u
     return HS_OK;
 }
 
-// Host-side state after a launched step: pending friction and pool parity.
-void advance_state(Groom& g, bool splat, float friction)
+// Host-side state after a launched step: pending friction, pool parity and
+// the tile shape its boxes were written in.
+void advance_state(Groom& g, bool splat, float friction, int shape)
 {
     g.steps += 1;
     if (splat) {
+        g.last_shape = shape;
         g.parity ^= 1u;
         g.pending = true;
         g.pend_mu = friction;
@@ -481,7 +499,7 @@ hipError_t enqueue_batch(const hs_ctx* c, const std::vector<BatchLaunch>& launch
     e = launch_mesh_pass(c, mp, st);
     if (e != hipSuccess) return e;
     for (const BatchLaunch& L : launches) {
-        e = hs::launch_step_batch(reinterpret_cast<const hs::StepArgs*>(c->dev_blob + L.args_off),
+        e = hs::launch_step_batch(c->shape, reinterpret_cast<const hs::StepArgs*>(c->dev_blob + L.args_off),
                                   reinterpret_cast<const uint32_t*>(c->dev_blob + L.wg_off), L.ng, L.total_wg,
                                   (L.v & 1) != 0, (L.v & 2) != 0, collider_mode(L.v), c->tile.xcd_order, st);
         if (e != hipSuccess) return e;
@@ -674,7 +692,7 @@ int check_groom_desc(const hs_groom_desc* d, const float* aos, std::string* why)
     if (d->num_points < 2 || d->num_points > HS_MAX_POINTS) return bad("num_points must be in [2, 256]");
     if (d->num_strands > (1u << 26)) return bad("too many strands");
     // kernels address the [3][N][Sp] state with 32-bit byte offsets: 12*N*Sp < 2^32
-    if ((uint64_t)d->num_points * round_up(d->num_strands, hs::kW) * 12 >= (1ull << 32))
+    if ((uint64_t)d->num_points * round_up(d->num_strands, hs::kMaxTileStrands) * 12 >= (1ull << 32))
         return bad("more than 357M points in one groom");
     if (d->grid_dim > 1024) return bad("grid_dim > 1024");
     if (d->grid_dim > 0 && !(d->grid_h > 0.0f && std::isfinite(d->grid_h)))
@@ -914,13 +932,13 @@ int hs_step(hs_ctx* c, uint32_t id, const hs_time* t, const hs_collider* cols, u
     }
     if (mesh) HS_TRY(c, hs::launch_mesh_force(a, c->stream));
     if (ev) HS_TRY(c, hipEventRecord(ev->e[0], c->stream));
-    HS_TRY(c, hs::launch_step(a, gather, splat, c->stream));
+    HS_TRY(c, hs::launch_step(c->shape, a, gather, splat, c->stream));
     if (ev) HS_TRY(c, hipEventRecord(ev->e[1], c->stream));
     if (splat) {
         HS_TRY(c, hs::launch_reduce(g->gw, g->G, g->parity, c->stream));
         if (ev) HS_TRY(c, hipEventRecord(ev->e[2], c->stream));
     }
-    advance_state(*g, splat, p->friction);
+    advance_state(*g, splat, p->friction, c->shape);
     return HS_OK;
 }
 
@@ -1024,6 +1042,7 @@ int hs_step_batch(hs_ctx* c, const hs_batch_item* items, uint32_t n)
     mp.blk_off = args_bytes + ell_bytes + mp.blk_off * sizeof(uint32_t);
     key.push_back(bytes);
     key.push_back(c->tile.xcd_order);
+    key.push_back((uint64_t)c->shape);  // the captured kernels are the tile shape's
     key.push_back(mp.nm);
     int rc = ensure_blob(c, bytes);
     if (rc != HS_OK) return rc;
@@ -1076,7 +1095,7 @@ int hs_step_batch(hs_ctx* c, const hs_batch_item* items, uint32_t n)
         }
         HS_TRY(c, hipEventRecord(ev->e[0], c->stream));
         for (const BatchLaunch& L : launches)
-            HS_TRY(c, hs::launch_step_batch(reinterpret_cast<const hs::StepArgs*>(c->dev_blob + L.args_off),
+            HS_TRY(c, hs::launch_step_batch(c->shape, reinterpret_cast<const hs::StepArgs*>(c->dev_blob + L.args_off),
                                             reinterpret_cast<const uint32_t*>(c->dev_blob + L.wg_off), L.ng,
                                             L.total_wg, (L.v & 1) != 0, (L.v & 2) != 0, collider_mode(L.v),
                                             c->tile.xcd_order, c->stream));
@@ -1096,7 +1115,8 @@ int hs_step_batch(hs_ctx* c, const hs_batch_item* items, uint32_t n)
     HS_TRY(c, hipEventRecord(c->slot_done[slot], c->stream));
     c->slot_used[slot] = true;
     c->batch_calls += 1;
-    for (uint32_t k = 0; k < n; ++k) advance_state(*gs[k], (variant[k] & 2) != 0, items[k].params->friction);
+    for (uint32_t k = 0; k < n; ++k)
+        advance_state(*gs[k], (variant[k] & 2) != 0, items[k].params->friction, c->shape);
     return HS_OK;
 }
 
@@ -1119,7 +1139,14 @@ int hs_set_tile(hs_ctx* c, const hs_tile* t)
     if (t->gather_cells < 64 || t->gather_cells > hs::kWinCells || t->splat_cells < 64 ||
         t->splat_cells > hs::kWinCells || t->xcd_order > 1)
         return fail(c, HS_ERR_INVALID, "hs_set_tile: window sizes must be in [64, 2048], xcd_order 0 or 1");
+    int shape = -1;
+    for (int k = 0; k < hs::kNumShapes; ++k)
+        if (hs::kShapes[k].strands == t->strands && hs::kShapes[k].points == t->points) shape = k;
+    if (shape < 0)
+        return fail(c, HS_ERR_INVALID,
+                    "hs_set_tile: (strands, points) must be one of the built shapes (128, 16), (256, 8), (256, 12)");
     c->tile = *t;
+    c->shape = shape;
     return HS_OK;
 }
 
@@ -1136,9 +1163,13 @@ int hs_autotune_batch(hs_ctx* c, const hs_batch_item* items, uint32_t n, uint32_
     if (!c) return HS_ERR_INVALID;
     if (!items || n == 0 || steps == 0) return fail(c, HS_ERR_INVALID, "hs_autotune_batch: empty batch");
     HS_TRY(c, hipSetDevice(c->device));
-    // copies of the grooms: the candidates run on these
+    // copies of the grooms: the candidates run on these, each from the
+    // grooms' present state (restored before every candidate, so all of
+    // them time the same steps)
     std::vector<hs_batch_item> tmp(items, items + n);
     std::vector<uint32_t> clones;
+    std::vector<const Groom*> srcs;
+    std::vector<Groom> start;  // each copy's host state at the start
     auto drop = [&] {
         (void)hipStreamSynchronize(c->stream);
         for (uint32_t id : clones) {
@@ -1165,10 +1196,22 @@ int hs_autotune_batch(hs_ctx* c, const hs_batch_item* items, uint32_t n, uint32_
             drop();
             return hip_fail(c, e, "hs_autotune_batch: copy");
         }
+        srcs.push_back(src);
+        start.push_back(g);
         const uint32_t id = ctx_adopt(c, std::move(g));
         clones.push_back(id);
         tmp[k].groom_id = id;
     }
+    auto restore = [&]() -> int {
+        for (uint32_t k = 0; k < n; ++k) {
+            Groom* cl = find(c, clones[k]);
+            *cl = start[k];
+            const hipError_t e =
+                hipMemcpyAsync(cl->block, srcs[k]->block, cl->bytes, hipMemcpyDeviceToDevice, c->stream);
+            if (e != hipSuccess) return hip_fail(c, e, "hs_autotune_batch: restore");
+        }
+        return HS_OK;
+    };
     const hs_tile saved = c->tile;
     const bool graphs = c->graphs;
     const bool prof = c->profile;
@@ -1177,28 +1220,36 @@ int hs_autotune_batch(hs_ctx* c, const hs_batch_item* items, uint32_t n, uint32_
     int rc = HS_OK;
     double best_t = 1e300;
     hs_tile best_tile = saved;
+    int best_shape = c->shape;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = HS_ERR_HIP;
+    const int saved_shape = c->shape;
     const uint32_t gcand[] = {512, 1024, 2048};
     const uint32_t scand[] = {512, 1024, 2048};
-    for (uint32_t gc : gcand)
-        for (uint32_t sc : scand)
-            for (uint32_t xo = 0; xo < 2 && rc == HS_OK; ++xo) {
-                if (gc > hs::kWinCells || sc > hs::kWinCells) continue;
-                c->tile = hs_tile{gc, sc, xo};
-                rc = hs_step_batch(c, tmp.data(), n);
+    for (int sh = 0; sh < hs::kNumShapes; ++sh)
+        for (uint32_t gc : gcand)
+            for (uint32_t sc : scand)
+                for (uint32_t xo = 0; xo < 2 && rc == HS_OK; ++xo) {
+                    if (gc > hs::kWinCells || sc > hs::kWinCells) continue;
+                    c->tile = hs_tile{gc, sc, xo, hs::kShapes[sh].strands, hs::kShapes[sh].points};
+                    c->shape = sh;
+                    // the untimed step also takes the change of shape (its
+                    // gather reads the float grid directly)
+                    rc = restore();
+                    if (rc == HS_OK) rc = hs_step_batch(c, tmp.data(), n);
                 if (rc != HS_OK) break;
-                if (hipEventRecord(e0, c->stream) != hipSuccess) rc = HS_ERR_HIP;
-                for (uint32_t s = 0; s < steps && rc == HS_OK; ++s) rc = hs_step_batch(c, tmp.data(), n);
-                float ms = 0.0f;
-                if (rc == HS_OK && (hipEventRecord(e1, c->stream) != hipSuccess ||
-                                    hipEventSynchronize(e1) != hipSuccess ||
-                                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-                    rc = HS_ERR_HIP;
-                if (rc == HS_OK && ms / steps < best_t) {
-                    best_t = ms / steps;
-                    best_tile = c->tile;
+                    if (hipEventRecord(e0, c->stream) != hipSuccess) rc = HS_ERR_HIP;
+                    for (uint32_t s = 0; s < steps && rc == HS_OK; ++s) rc = hs_step_batch(c, tmp.data(), n);
+                    float ms = 0.0f;
+                    if (rc == HS_OK && (hipEventRecord(e1, c->stream) != hipSuccess ||
+                                        hipEventSynchronize(e1) != hipSuccess ||
+                                        hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                        rc = HS_ERR_HIP;
+                    if (rc == HS_OK && ms / steps < best_t) {
+                        best_t = ms / steps;
+                        best_tile = c->tile;
+                        best_shape = sh;
+                    }
                 }
-            }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     c->profile = prof;
@@ -1206,9 +1257,11 @@ int hs_autotune_batch(hs_ctx* c, const hs_batch_item* items, uint32_t n, uint32_
     drop();
     if (rc != HS_OK) {
         c->tile = saved;
+        c->shape = saved_shape;
         return rc;
     }
     c->tile = best_tile;
+    c->shape = best_shape;
     if (best) *best = best_tile;
     if (best_ms) *best_ms = best_t;
     return HS_OK;

@@ -32,10 +32,19 @@ struct DevHeap {
 };
 
 // Shape-derived sizes of one groom's device state.
+// Strand planes padded to Sp slots; the grid workspace holds the pool slots,
+// descriptors and deferred-list entries of the tile shape with the most
+// workgroup-segments (nws_cap).
 struct Layout {
-    uint32_t Sp, nwg, nseg;
+    uint32_t Sp, nws_cap;
     uint64_t pool_cap, desc_cap;
 };
+
+// Workgroups and segments of an S x N groom in tile shape hs::kShapes[shape].
+struct ShapeDims {
+    uint32_t nwg, nseg;
+};
+ShapeDims shape_dims(int shape, uint32_t S, uint32_t N);
 
 Layout layout_of(uint32_t S, uint32_t N, uint32_t G);
 
@@ -61,6 +70,7 @@ struct Groom {
     float pend_mu = 0.0f, pend_omu = 1.0f;
     uint32_t parity = 0;           // pool/counter half the next splat writes
     uint32_t steps = 0;            // steps launched (the kernels' list tag)
+    int last_shape = -1;           // tile shape of the last splat step (its gbox layout)
     uint64_t bytes = 0;            // size of `block`
 };
 
@@ -140,7 +150,8 @@ struct hs_ctx {
     std::vector<BatchGraph> graph_cache;
     uint64_t batch_calls = 0;
     uint64_t graph_captures = 0;
-    hs_tile tile{hs::kWinCells, hs::kWinCells, 1};
+    hs_tile tile{hs::kWinCells, hs::kWinCells, 1, hs::kShapes[0].strands, hs::kShapes[0].points};
+    int shape = 0;  // index of (tile.strands, tile.points) in hs::kShapes
 };
 
 namespace hsi {

@@ -8,19 +8,22 @@
 
 namespace hs {
 
-// Tile shape of the step kernel (compile-time tunables; the defaults are the
-// measured best on MI355X for the 100k x 32 workload).
-#ifndef HS_TILE_STRANDS
-#define HS_TILE_STRANDS 128
-#endif
-#ifndef HS_TILE_POINTS
-#define HS_TILE_POINTS 16
-#endif
+// Tile shapes the step kernels are built for (hs_step.hip, once per shape):
+// strands per workgroup (one lane each) x points per segment of the
+// root-to-tip sweep.  Results are identical for every shape.  Strands are
+// ordered in k-d leaves of kLeafStrands; a workgroup covers whole leaves, and
+// the strand planes are padded to a multiple of kMaxTileStrands so every shape
+// fits the same layout.
+struct TileShape {
+    uint32_t strands, points;
+};
+constexpr TileShape kShapes[] = {{128, 16}, {256, 8}, {256, 12}};
+constexpr int kNumShapes = 3;
+constexpr uint32_t kLeafStrands = 128;
+constexpr uint32_t kMaxTileStrands = 256;
 #ifndef HS_WINDOW_CELLS
 #define HS_WINDOW_CELLS 2048
 #endif
-constexpr uint32_t kW = HS_TILE_STRANDS;         // strands per step workgroup
-constexpr uint32_t kL = HS_TILE_POINTS;          // points per segment of the root-to-tip sweep
 constexpr uint32_t kWinCells = HS_WINDOW_CELLS;  // LDS window of one workgroup, 16 B entries
 // reduce kernel: bricks of kBX x kBY x kBZ voxels, one wavefront each
 constexpr uint32_t kBX = 8, kBY = 4, kBZ = 2;
@@ -40,21 +43,23 @@ struct GridWork {
     int* desc;            // partial-box descriptors, [2][desc_cap][kDescInts]
     uint32_t* counters;   // [2][4]: shared cells used, shared boxes used, fallback flag, deferred entries
     int4* overflow;       // G^3 int4 atomic fallback when the pool is full (normally zero)
-    int* gbox;            // [nwg][nseg][6] voxel box each workgroup-segment touched last step
+    int* gbox;            // [nwg][nseg][6] voxel box each workgroup-segment touched last step (laid
+                          // out for the last step's tile shape)
     uint32_t* brick_cnt;  // [2][nbricks] boxes listed per brick (may exceed kBrickList)
     uint32_t* brick_list; // [2][nbricks][kBrickList] indices of the boxes overlapping a brick
     uint32_t* brick_nz;   // [nbricks] 1 when the brick's grid cells are not all zero
     unsigned long long* near_limit;  // (cell, step) pairs with a component >= 2^30 in magnitude
     // workgroup-segments whose box needs several windows, splatted chunk by
     // chunk by any workgroup of the groom that is done with its own strands
-    unsigned long long* dlist;  // [2][nws] step tag (28 bits) | chunks (12) | workgroup-segment (24)
-    uint32_t* dclaim;           // [2][nws] chunks taken of each list entry
-    int* dbox;                  // [nws][kDBoxInts] box and chunking of a deferred segment
+    unsigned long long* dlist;  // [2][nws_cap] step tag (28 bits) | chunks (12) | workgroup-segment (24)
+    uint32_t* dclaim;           // [2][nws_cap] chunks taken of each list entry
+    int* dbox;                  // [nws_cap][kDBoxInts] box and chunking of a deferred segment
     // pool / descriptors per parity: first each workgroup-segment's own slot
     // (kSlotCells cells, descriptor index = its number), then a shared part
     // taken with atomics; pool_cap / desc_cap count both
     uint32_t pool_cap, desc_cap;
-    uint32_t nws;            // workgroup-segments: owners of the slots
+    uint32_t nws;            // workgroup-segments of the launch's tile shape: owners of the slots
+    uint32_t nws_cap;        // the most any tile shape has: the stride of dlist / dclaim halves
     uint32_t nbx, nby, nbz;  // reduce bricks per axis
 };
 
@@ -111,6 +116,7 @@ struct StepArgs {
     uint32_t win_gather;  // voxel cells of the gather window (<= kWinCells)
     uint32_t win_splat;   // voxel cells of a splat window (<= kWinCells)
     uint32_t swz;         // 1: XCD-aware workgroup order
+    uint32_t gbox_ok;     // 1: gw.gbox holds the last step's boxes in this launch's tile shape
 };
 
 struct UnpackGrid {
@@ -119,7 +125,39 @@ struct UnpackGrid {
     float ox, oy, oz, inv_h, mu, omu;
 };
 
-hipError_t launch_step(const StepArgs& a, bool gather, bool splat, hipStream_t st);
+// Step kernels of each tile shape (namespace t<strands>x<points>).
+#define HS_SHAPE_LAUNCHERS(NS)                                                                                   \
+    namespace NS {                                                                                               \
+    hipError_t launch_step(const StepArgs& a, bool gather, bool splat, hipStream_t st);                          \
+    /* many grooms per launch: per-groom arguments in device memory; wg_start is the prefix sum (ngroom + 1) */ \
+    hipError_t launch_step_batch(const StepArgs* args_dev, const uint32_t* wg_start_dev, uint32_t ngroom,        \
+                                 uint32_t total_wg, bool gather, bool splat, int col, uint32_t swz,              \
+                                 hipStream_t st);                                                                \
+    }
+HS_SHAPE_LAUNCHERS(t128x16)
+HS_SHAPE_LAUNCHERS(t256x8)
+HS_SHAPE_LAUNCHERS(t256x12)
+#undef HS_SHAPE_LAUNCHERS
+
+// The step launchers of kShapes[shape].
+inline hipError_t launch_step(int shape, const StepArgs& a, bool gather, bool splat, hipStream_t st)
+{
+    switch (shape) {
+    case 1: return t256x8::launch_step(a, gather, splat, st);
+    case 2: return t256x12::launch_step(a, gather, splat, st);
+    default: return t128x16::launch_step(a, gather, splat, st);
+    }
+}
+inline hipError_t launch_step_batch(int shape, const StepArgs* args_dev, const uint32_t* wg_start_dev,
+                                    uint32_t ngroom, uint32_t total_wg, bool gather, bool splat, int col,
+                                    uint32_t swz, hipStream_t st)
+{
+    switch (shape) {
+    case 1: return t256x8::launch_step_batch(args_dev, wg_start_dev, ngroom, total_wg, gather, splat, col, swz, st);
+    case 2: return t256x12::launch_step_batch(args_dev, wg_start_dev, ngroom, total_wg, gather, splat, col, swz, st);
+    default: return t128x16::launch_step_batch(args_dev, wg_start_dev, ngroom, total_wg, gather, splat, col, swz, st);
+    }
+}
 hipError_t launch_reduce(const GridWork& gw, uint32_t G, uint32_t parity, hipStream_t st);
 // Mesh pass: mesh contact of every stored position of `a` into a.mesh_f
 // (N * Sp float4).  Runs before a step kernel that reads it.
@@ -128,10 +166,8 @@ hipError_t launch_mesh_force(const StepArgs& a, hipStream_t st);
 // holds prefix sums of ceil(N * Sp / 256) blocks (nm + 1 entries).
 hipError_t launch_mesh_force_batch(const StepArgs* args_dev, const uint32_t* idx_dev, const uint32_t* blk_start_dev,
                                    uint32_t nm, uint32_t total_blocks, hipStream_t st);
-// Many grooms per launch: per-groom arguments in device memory; wg_start /
-// brick_start are prefix sums (ngroom + 1 entries) of workgroups / bricks.
-hipError_t launch_step_batch(const StepArgs* args_dev, const uint32_t* wg_start_dev, uint32_t ngroom,
-                             uint32_t total_wg, bool gather, bool splat, int col, uint32_t swz, hipStream_t st);
+// Reduce of many grooms per launch: brick_start is the prefix sum (ngroom + 1
+// entries) of their bricks.
 hipError_t launch_reduce_batch(const StepArgs* args_dev, const uint32_t* brick_start_dev, uint32_t ngroom,
                                uint32_t total_bricks, hipStream_t st);
 hipError_t launch_pack(const float* aos_dev, const uint32_t* perm, uint32_t S, uint32_t N, uint32_t Sp,

@@ -105,7 +105,12 @@ class HsMeshDesc(C.Structure):
 
 
 class HsTile(C.Structure):
-    _fields_ = [("gather_cells", C.c_uint32), ("splat_cells", C.c_uint32), ("xcd_order", C.c_uint32)]
+    _fields_ = [("gather_cells", C.c_uint32), ("splat_cells", C.c_uint32), ("xcd_order", C.c_uint32),
+                ("strands", C.c_uint32), ("points", C.c_uint32)]
+
+
+# (strands per workgroup, points per segment) of the built step kernels
+TILE_SHAPES = ((128, 16), (256, 8), (256, 12))
 
 
 class HsBatchItem(C.Structure):
@@ -359,21 +364,23 @@ class Context:
     def batch(self, grooms, max_colliders=1):
         return Batch(self, grooms, max_colliders)
 
-    def set_tile(self, gather_cells=2048, splat_cells=2048, xcd_order=1):
-        self._check(lib().hs_set_tile(self._h, C.byref(HsTile(gather_cells, splat_cells, xcd_order))))
+    def set_tile(self, gather_cells=2048, splat_cells=2048, xcd_order=1, strands=128, points=16):
+        t = HsTile(gather_cells, splat_cells, xcd_order, strands, points)
+        self._check(lib().hs_set_tile(self._h, C.byref(t)))
 
     def get_tile(self):
+        """(gather_cells, splat_cells, xcd_order, strands, points)"""
         t = HsTile()
         self._check(lib().hs_get_tile(self._h, C.byref(t)))
-        return (t.gather_cells, t.splat_cells, t.xcd_order)
+        return (t.gather_cells, t.splat_cells, t.xcd_order, t.strands, t.points)
 
     def autotune(self, batch, steps=3):
         """Time every tile candidate on copies of `batch`'s grooms; keep the fastest.
-        Returns ((gather_cells, splat_cells, xcd_order), ms per batched step)."""
+        Returns ((gather_cells, splat_cells, xcd_order, strands, points), ms per batched step)."""
         t = HsTile()
         ms = C.c_double()
         self._check(lib().hs_autotune_batch(self._h, batch._items, batch.n, steps, C.byref(t), C.byref(ms)))
-        return (t.gather_cells, t.splat_cells, t.xcd_order), ms.value
+        return (t.gather_cells, t.splat_cells, t.xcd_order, t.strands, t.points), ms.value
 
     def mesh(self, verts, tris, grid_dim=(16, 16, 16), margin=0.05):
         """Register a closed triangle mesh collider; returns its id for step_params(mesh_id=...)."""

@@ -60,8 +60,9 @@ def velocities(aos, N):
 
 
 def run_parity(ctx, wl, steps, check_every=1, groom=0, keep_corrections=False, ncol=1, colliders_fn=None,
-               params_kw=None, check_grid=True):
-    """Step GPU and ref32 side by side; assert bitwise equality of strands (and grid)."""
+               params_kw=None, check_grid=True, tile_fn=None):
+    """Step GPU and ref32 side by side; assert bitwise equality of strands (and grid).
+    tile_fn(k): the tile (set_tile arguments) to step k with, or None to keep the current one."""
     g = wl.grooms[groom]
     gpu = make_gpu_groom(ctx, wl, groom, keep_corrections=keep_corrections)
     ref = g.strands.copy()
@@ -69,6 +70,8 @@ def run_parity(ctx, wl, steps, check_every=1, groom=0, keep_corrections=False, n
     params_kw = params_kw or {}
     try:
         for k in range(steps):
+            if tile_fn is not None and tile_fn(k) is not None:
+                ctx.set_tile(*tile_fn(k))
             cols = colliders_fn(k) if colliders_fn else [scenes.sphere_collider(wl.sphere(k, groom))]
             gpu.step(wl.dt, float(wl.time(k)), cols, gpu_params(wl, k, groom, ncol=ncol, **params_kw))
             op = oracle_params(wl, k, groom, colliders=np.stack(cols)[:ncol] if ncol else None, **params_kw)

@@ -182,14 +182,56 @@ def test_autotune_leaves_state_and_results_unchanged(ctx):
         g.destroy()
 
 
-@pytest.mark.parametrize("tile", [(512, 512, 0), (1024, 2048, 1), (2048, 512, 0), (64, 64, 1)])
+@pytest.mark.parametrize("tile", [(512, 512, 0), (1024, 2048, 1), (2048, 512, 0), (64, 64, 1),
+                                  (2048, 2048, 1, 256, 8), (1024, 512, 0, 256, 12), (64, 64, 1, 256, 8),
+                                  (2048, 1024, 1, 256, 12)])
 def test_tile_choices_bitwise(ctx, tile):
     wl = scenes.plain_100k(S=3000, N=20, G=32)
     ctx.set_tile(*tile)
     try:
+        assert ctx.get_tile()[:len(tile)] == tile
         run_parity(ctx, wl, 5)
     finally:
         ctx.set_tile(2048, 2048, 1)
+
+
+def test_tile_shape_changes_bitwise(ctx):
+    """The tile shape may change between steps: the step after a change
+    gathers from the float grid (the last step's boxes are laid out for the
+    old shape) and the deferred-list halves keep one stride, so strands and
+    voxel ints stay equal to ref32.  3000 strands (not a multiple of 256) x 20
+    points: 24 or 12 workgroups, 2 or 3 segments."""
+    wl = scenes.plain_100k(S=3000, N=20, G=32)
+    shapes = hairsim.TILE_SHAPES
+    order = [0, 1, 2, 0, 2, 1, 1, 0]
+    try:
+        run_parity(ctx, wl, len(order), tile_fn=lambda k: (2048, 1024 if k % 2 else 2048, k % 2) + shapes[order[k]])
+        # and in a batch, with small splat windows (many deferred segments)
+        bw = small_batch(n=5, seed=61)
+        ctx.set_tile(2048, 2048, 1)
+        alone = [run_parity(ctx, bw, 6, groom=gi)[0] for gi in range(len(bw.grooms))]
+        grs = [make_gpu_groom(ctx, bw, gi) for gi in range(len(bw.grooms))]
+        for k in range(6):
+            ctx.set_tile(2048, 256 if k % 2 else 2048, 1, *shapes[order[k]])
+            ctx.step_batch([(gr, bw.dt, float(bw.time(k)), [scenes.sphere_collider(bw.sphere(k, gi))],
+                             gpu_params(bw, k, gi)) for gi, gr in enumerate(grs)])
+        for gi, gr in enumerate(grs):
+            o = gr.read()[0].reshape(-1, 3, bw.grooms[gi].N, 4).copy()
+            r = alone[gi].reshape(-1, 3, bw.grooms[gi].N, 4).copy()
+            o[:, 2] = 0
+            r[:, 2] = 0
+            assert_bitwise(o, r, f"batched groom {gi} with shape changes vs ref32", bw.grooms[gi].N)
+            gr.destroy()
+    finally:
+        ctx.set_tile(2048, 2048, 1)
+
+
+def test_set_tile_rejects_unbuilt_shape(ctx):
+    for bad in [(2048, 2048, 1, 64, 16), (2048, 2048, 1, 128, 8), (2048, 2048, 1, 256, 16), (2048, 2048, 1, 0, 0)]:
+        with pytest.raises(hairsim.HairSimError) as e:
+            ctx.set_tile(*bad)
+        assert e.value.code == hairsim.HS_ERR_INVALID
+    assert ctx.get_tile() == (2048, 2048, 1, 128, 16)
 
 
 def test_deferred_chunks_bitwise(ctx):
