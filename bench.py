@@ -180,7 +180,7 @@ def tune_tile(ctx, hairsim, scenes, wl, mesh_id, warmup):
     b._params[0] = make_params(hairsim, wl, warmup, mesh_id)
     b.set_colliders(scenes.sphere_collider(wl.sphere(warmup))[None, None])
     b.set_time(wl.dt, float(wl.time(warmup)))
-    tile, ms = ctx.autotune(b, steps=3)
+    tile, ms = ctx.autotune(b, steps=5)
     ctx.sync()
     groom.destroy()
     return tile, ms
@@ -199,8 +199,13 @@ def bench_single(args, hairsim, scenes, wl):
     mesh_id = 0
     if wl.mesh is not None:
         mesh_id = ctx.mesh(*wl.mesh, grid_dim=wl.mesh_grid, margin=wl.mesh_margin)
-    tile, tuned_ms = tune_tile(ctx, hairsim, scenes, wl, mesh_id, warmup) if not args.no_autotune \
-        else (ctx.get_tile(), None)
+    if args.tile:
+        ctx.set_tile(*args.tile)
+        tile, tuned_ms = ctx.get_tile(), None
+    elif not args.no_autotune:
+        tile, tuned_ms = tune_tile(ctx, hairsim, scenes, wl, mesh_id, warmup)
+    else:
+        tile, tuned_ms = ctx.get_tile(), None
     # the timed run, then the same scene steps again from a fresh copy of the
     # groom with kernel events: the kernel averages describe the timed steps
     el, _, out, near = run_scene(ctx, hairsim, scenes, wl, mesh_id, warmup, steps, profile=False)
@@ -313,7 +318,13 @@ def bench_batched(args, hairsim, scenes, wl):
     # tile choice: timed on copies of the grooms at the first timed step
     # (their state is untouched)
     b.set_time(wl.dt, float(wl.time(args.warmup)))
-    tile, tuned_ms = ctx.autotune(b, steps=3) if not args.no_autotune else (ctx.get_tile(), None)
+    if args.tile:
+        ctx.set_tile(*args.tile)
+        tile, tuned_ms = ctx.get_tile(), None
+    elif not args.no_autotune:
+        tile, tuned_ms = ctx.autotune(b, steps=5)
+    else:
+        tile, tuned_ms = ctx.get_tile(), None
     ctx.sync()
     t0 = time.perf_counter()
     for k in range(args.warmup, args.warmup + args.steps):
@@ -589,6 +600,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="batched: eager launches instead of graphs")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tile")
+    ap.add_argument("--tile", type=lambda v: tuple(int(x) for x in v.split(",")), default=None,
+                    help="use this tile instead of autotuning: gather_cells,splat_cells,xcd_order,strands,points")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="run only a plain elementwise kernel over MIB MiB (byte-counter calibration)")
     args = ap.parse_args()
