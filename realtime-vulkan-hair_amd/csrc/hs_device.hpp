@@ -13,6 +13,18 @@
 
 namespace hs {
 
+// floor(x / d) for x in [0, 2^16) and a workgroup-uniform d in [1, 2^16),
+// without a division per x.  For d >= 2, m = floor((2^32 - 1) / d) + 1 =
+// 2^32/d + e with 0 <= e <= 1 (e = 0 when d is a power of two) fits 32 bits,
+// and x*m / 2^32 exceeds x/d by less than x / 2^32 < 2^-16 < 1/d, which never
+// carries x/d past the next integer: the high word of x*m is floor(x/d).
+// d = 1 (m would be 2^32) is selected apart.
+__device__ __forceinline__ uint32_t div_magic(uint32_t d) { return d > 1 ? 0xffffffffu / d + 1u : 0u; }
+__device__ __forceinline__ uint32_t div_small(uint32_t x, uint32_t d, uint32_t m)
+{
+    return d > 1 ? __umulhi(x, m) : x;
+}
+
 // ---------------------------------------------------------------------------
 // Voxel ranges
 // ---------------------------------------------------------------------------
@@ -183,8 +195,8 @@ __device__ __forceinline__ f3 gather_any(const float4* win, const Win& wd, bool 
 // ---------------------------------------------------------------------------
 constexpr int kMaxCopies = 16;
 
-// Where a lane's splat goes: its copy and the corner it starts its 2x2x2
-// block at (lanes that share a copy start at different corners).  Copy k of
+// Where a lane's splat goes: its copy and the order it visits the corners of
+// its 2x2x2 block in (lanes that share a copy take different orders).  Copy k of
 // window voxel w is entry w * nc + k.  (Copy-major orders and separate
 // planes per component were measured no faster.)
 struct SplatLane {
@@ -240,15 +252,23 @@ __device__ __forceinline__ void splat_block(int4* win, const Win& wd, const Rng&
     const int sx = sl.nc, sy = sx * wd.ex, sz = sy * wd.ey;
     const int e0 = splat_entry(win_index(wd, r.x0, r.y0, r.z0), sl);
     unsigned char* wb = reinterpret_cast<unsigned char*>(win);
-    // Lanes that share a copy start at different corners, so lanes of one
-    // voxel block hit different cells per instruction.  Integer adds commute
-    // and each corner's weight is still (wx * wy) * wz: the sums are unchanged.
+    // Lanes that share a copy visit the corners in different orders (step i
+    // takes corner i ^ rot), so lanes of one voxel block hit different cells
+    // per instruction.  The lane's order is set up once: per axis the weight
+    // and entry offset of step bit 0 and 1 are swapped when rot has that bit,
+    // and every step then takes fixed registers.  Integer adds commute and
+    // each corner's weight is still (wx * wy) * wz: the sums are unchanged.
+    const bool fx = (sl.rot & 1) != 0, fy = (sl.rot & 2) != 0, fz = (sl.rot & 4) != 0;
+    const float xa = fx ? wx1 : wx0, xb = fx ? wx0 : wx1;
+    const float ya = fy ? wy1 : wy0, yb = fy ? wy0 : wy1;
+    const float za = fz ? wz1 : wz0, zb = fz ? wz0 : wz1;
+    const int ea = e0 + (fx ? sx : 0) + (fy ? sy : 0) + (fz ? sz : 0);  // step 0's corner
+    const int dx = fx ? -sx : sx, dy = fy ? -sy : sy, dz = fz ? -sz : sz;
+    const float wxy[4] = {xa * ya, xb * ya, xa * yb, xb * yb};
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        const int k = (i + sl.rot) & 7;
-        const bool dx = (k & 1) != 0, dy = (k & 2) != 0, dz = (k & 4) != 0;
-        const float w = ((dx ? wx1 : wx0) * (dy ? wy1 : wy0)) * (dz ? wz1 : wz0);
-        splat_cell(wb, e0 + (dx ? sx : 0) + (dy ? sy : 0) + (dz ? sz : 0), w, v);
+        const float w = wxy[i & 3] * ((i & 4) ? zb : za);
+        splat_cell(wb, ea + ((i & 1) ? dx : 0) + ((i & 2) ? dy : 0) + ((i & 4) ? dz : 0), w, v);
     }
 }
 

@@ -137,7 +137,7 @@ struct Plan {
     bool ok;          // false: no chunking fits (or empty box)
 };
 
-__device__ __forceinline__ Plan plan_box(const Box& b, int cap, bool splat_plan = false)
+__device__ __forceinline__ Plan plan_box(const Box& b, int cap)
 {
     Plan p;
     if (box_is_empty(b)) {
@@ -154,11 +154,6 @@ __device__ __forceinline__ Plan plan_box(const Box& b, int cap, bool splat_plan 
     if (p.ex * p.ey * p.ez <= cap) {
         p.cy = p.ey;
         p.cz = p.ez;
-#ifdef HS_NO_CHUNKS
-    } else if (splat_plan) {
-        p.ok = false;
-        p.cy = p.cz = 2;
-#endif
     } else if (p.ex * p.ey * 2 <= cap) {
         p.cy = p.ey;
         p.cz = cap / (p.ex * p.ey);
@@ -529,16 +524,20 @@ __device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
                 float4* fw = reinterpret_cast<float4*>(win);
                 // loads for up to 8 cells per thread in flight before the LDS stores
                 const uint32_t GG = (uint32_t)G * (uint32_t)G;
+                const uint32_t ex = (uint32_t)gwin.ex, ey = (uint32_t)gwin.ey;
+                const uint32_t mx = div_magic(ex), my = div_magic(ey);
                 for (int c0 = 0; c0 < cells; c0 += 8 * kW) {
                     // unconditional (clamped) loads: all 8 in flight before the stores
                     float4 tmp[8];
 #pragma unroll
                     for (int k = 0; k < 8; ++k) {
                         const uint32_t c = (uint32_t)min(c0 + k * (int)kW + (int)threadIdx.x, cells - 1);
-                        const uint32_t ax = c % (uint32_t)gwin.ex;
-                        const uint32_t rest = c / (uint32_t)gwin.ex;
-                        const uint32_t gi = (uint32_t)gwin.x0 + ax + ((uint32_t)gwin.y0 + rest % (uint32_t)gwin.ey) * (uint32_t)G +
-                                            ((uint32_t)gwin.z0 + rest / (uint32_t)gwin.ey) * GG;
+                        const uint32_t rest = div_small(c, ex, mx);
+                        const uint32_t ax = c - rest * ex;
+                        const uint32_t az = div_small(rest, ey, my);
+                        const uint32_t ay = rest - az * ey;
+                        const uint32_t gi = (uint32_t)gwin.x0 + ax + ((uint32_t)gwin.y0 + ay) * (uint32_t)G +
+                                            ((uint32_t)gwin.z0 + az) * GG;
                         tmp[k] = a.gw.grid_f[gi];
                     }
 #pragma unroll
@@ -664,7 +663,7 @@ __device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
                 }
             }
             const Box box = block_box(sbox, red);
-            const Plan pl = plan_box(box, min((int)a.win_splat, (int)kWinCells), true);
+            const Plan pl = plan_box(box, min((int)a.win_splat, (int)kWinCells));
             uint32_t* ctr = a.gw.counters + a.parity * 4;
             if (!box_is_empty(box) && (!pl.ok || pl.ny * pl.nz > (int)kMaxDeferredChunks)) {
                 // no window fits (or more chunks than a list entry counts):
