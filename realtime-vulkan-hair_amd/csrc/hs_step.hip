@@ -94,37 +94,40 @@ __device__ __forceinline__ bool box_is_empty(const Box& b)
     return b.lo[0] > b.hi[0] || b.lo[1] > b.hi[1] || b.lo[2] > b.hi[2];
 }
 
-// Reduce a box over the workgroup; every thread gets the result.
-__device__ __forceinline__ Box block_box(Box b, int* scratch)
+// Reduce two boxes over the workgroup in one pass (one pair of barriers);
+// every thread gets both results.  `scratch` holds kWaves * 12 ints.
+__device__ __forceinline__ void block_box2(const Box& b0, const Box& b1, int* scratch, Box* r0, Box* r1)
 {
-    int v[6];
+    int v[12];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        v[k] = b.lo[k];
-        v[k + 3] = b.hi[k] == INT32_MIN ? INT32_MAX : -b.hi[k];
+        v[k] = b0.lo[k];
+        v[k + 3] = b0.hi[k] == INT32_MIN ? INT32_MAX : -b0.hi[k];
+        v[k + 6] = b1.lo[k];
+        v[k + 9] = b1.hi[k] == INT32_MIN ? INT32_MAX : -b1.hi[k];
     }
 #pragma unroll
-    for (int k = 0; k < 6; ++k) v[k] = wave_min(v[k]);
+    for (int k = 0; k < 12; ++k) v[k] = wave_min(v[k]);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (lane == 0) {
 #pragma unroll
-        for (int k = 0; k < 6; ++k) scratch[wave * 6 + k] = v[k];
+        for (int k = 0; k < 12; ++k) scratch[wave * 12 + k] = v[k];
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
+    for (int k = 0; k < 12; ++k) {
         int m = scratch[k];
-        for (int w = 1; w < kWaves; ++w) m = min(m, scratch[w * 6 + k]);
+        for (int w = 1; w < kWaves; ++w) m = min(m, scratch[w * 12 + k]);
         v[k] = m;
     }
     __syncthreads();
-    Box r;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        r.lo[k] = v[k];
-        r.hi[k] = v[k + 3] == INT32_MAX ? INT32_MIN : -v[k + 3];
+        r0->lo[k] = v[k];
+        r0->hi[k] = v[k + 3] == INT32_MAX ? INT32_MIN : -v[k + 3];
+        r1->lo[k] = v[k + 6];
+        r1->hi[k] = v[k + 9] == INT32_MAX ? INT32_MIN : -v[k + 9];
     }
-    return r;
 }
 
 // Chunking of a box into LDS windows of at most `cap` voxels: full x extent,
@@ -456,7 +459,7 @@ template <bool GATHER, bool SPLAT, int COL>
 __device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
 {
     __shared__ __align__(16) int4 win[kWinCells];
-    __shared__ int red[kW > kWaves * 6 ? kW : kWaves * 6];  // box reductions; the deferred-list scan
+    __shared__ int red[kW > kWaves * 12 ? kW : kWaves * 12];  // box reductions; the deferred-list scan
     __shared__ uint32_t bcast[3];
     __shared__ uint32_t own_k[HS_MAX_OWN_DEFERRED];  // deferred-list entries this workgroup published
     __shared__ uint32_t n_own[1];
@@ -653,7 +656,8 @@ __device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
                     if (t + 1 < cnt || (t + 1 == cnt && last_seg))
                         box_add(sbox, RP[t] ? unpack_block(RP[t]) : point_range(grid_coord(X[t], origin, a.inv_h), G));
             }
-            const Box gbn = block_box(nbox, red);
+            Box gbn, box;
+            block_box2(nbox, sbox, red, &gbn, &box);
             if (threadIdx.x == 0) {
                 int* gb = a.gw.gbox + ((size_t)wg * a.nseg + sg) * 6;
 #pragma unroll
@@ -662,7 +666,6 @@ __device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
                     gb[3 + k] = gbn.hi[k];
                 }
             }
-            const Box box = block_box(sbox, red);
             const Plan pl = plan_box(box, min((int)a.win_splat, (int)kWinCells));
             uint32_t* ctr = a.gw.counters + a.parity * 4;
             if (!box_is_empty(box) && (!pl.ok || pl.ny * pl.nz > (int)kMaxDeferredChunks)) {
