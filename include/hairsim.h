@@ -386,12 +386,14 @@ int hs_initial_strands(const float* roots, const float* root_normals, uint32_t n
 /* Served simulation jobs (no reference counterpart: the reference steps one  */
 /* groom per frame, main.cpp:257-285).  A server owns a context, an HBM arena */
 /* of `hbm_budget` bytes and three threads: admission (uploads jobs in FIFO   */
-/* order while the HBM budget allows), the step loop (each hs_step_batch      */
-/* takes resident jobs in round-robin order up to max_points_per_step and     */
-/* steps each once; it builds the next launch while the current one runs,    */
-/* with at most two launches queued) and completion (reads finished jobs      */
-/* back).  When the resident jobs hold more points than one launch, each     */
-/* job advances once every few launches.                                      */
+/* order while the HBM budget allows; each admitted job goes to the step      */
+/* loop with the fewest points), one or more step loops (each a context with  */
+/* its own stream: each hs_step_batch takes the loop's resident jobs in       */
+/* round-robin order up to max_points_per_step and steps each once; the loop  */
+/* builds its next launch while the current one runs, with at most           */
+/* launch_depth launches queued; the loops' launches run concurrently) and    */
+/* completion (reads finished jobs back).  When a loop's jobs hold more       */
+/* points than one launch, each advances once every few launches.             */
 /* ------------------------------------------------------------------------ */
 
 /* Wind and collider-0 centre at time t (seconds since the job's start),
@@ -420,11 +422,16 @@ typedef struct hs_job {
  * / hs_server_drain): the admission thread reads the first three when it
  * uploads the job, the step loop reads the keys every step. */
 
+#define HS_SERVER_MAX_STEP_LOOPS 8
 typedef struct hs_server_config {
     int device;
     uint64_t hbm_budget;          /* device bytes of resident job state         */
     uint64_t max_points_per_step; /* points in one batched launch (latency cap) */
     int use_graphs;               /* replay captured graphs for repeated shapes */
+    uint32_t step_loops;          /* step loops, each a context with its own    */
+                                  /* stream; their launches run concurrently    */
+                                  /* (0: 4, at most HS_SERVER_MAX_STEP_LOOPS)    */
+    uint32_t launch_depth;        /* launches queued per loop, 1 or 2 (0: 1)    */
 } hs_server_config;
 
 typedef struct hs_server_stats {
@@ -438,8 +445,9 @@ typedef struct hs_server_stats {
     double busy_s;                /* time with at least one job admitted        */
     uint64_t peak_resident_bytes;
     uint32_t peak_jobs_per_launch;
-    double gpu_busy_s;            /* sum over launches of their device time;    */
-                                  /* gpu_busy_s / busy_s is the busy fraction   */
+    double gpu_busy_s;            /* sum over launches of their device time     */
+                                  /* (with several step loops launches overlap, */
+                                  /* so it can exceed busy_s)                   */
     double p50_latency_ms, p99_latency_ms;  /* per job-step: host enqueue to    */
                                   /* completion seen by the host, the wait      */
                                   /* behind the launch queued before included   */

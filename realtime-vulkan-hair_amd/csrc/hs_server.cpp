@@ -114,6 +114,7 @@ struct Job {
     Clock::time_point t_submit;
     uint32_t done_steps = 0;
     uint32_t groom_id = 0;
+    struct Loop* loop = nullptr;  // the step loop (context) the job's groom lives in
     hsi::Groom g;
     void* staging = nullptr;
     JobState state = JobState::Queued;
@@ -159,27 +160,38 @@ struct Flight {
     std::vector<Job*> last;
 };
 
+// One step loop: a context (its own stream, argument blob and pinned slots)
+// and the thread that launches batched steps of the jobs assigned to it.
+// Loops run their launches concurrently on their own streams.
+constexpr int kMaxDepth = 2;  // launches queued per stream at once (the pinned argument slots)
+struct Loop {
+    hs_ctx* ctx = nullptr;
+    std::vector<Job*> ready;  // uploaded, joins a following launch
+    uint64_t points = 0;      // points of the jobs assigned to this loop
+    Flight flight[kMaxDepth];
+    std::thread t;
+};
+
 }  // namespace
 
 struct hs_server {
-    static constexpr int kLaunchDepth = 2;  // launches queued on the stream at once
     hs_server_config cfg{};
-    hs_ctx* ctx = nullptr;
+    int depth = 1;  // launches queued per loop
+    std::vector<std::unique_ptr<Loop>> loops;
     Arena arena;
     hipStream_t up = nullptr, down = nullptr;
-    Flight flight[kLaunchDepth];
 
     std::mutex m;
     std::condition_variable cv_admit, cv_loop, cv_read, cv_done;
     std::deque<Job*> queue;     // submitted, not admitted
-    std::vector<Job*> ready;    // uploaded, joins a following launch
     std::deque<Job*> finished;  // last step done, to be read back
     std::unordered_map<uint64_t, std::unique_ptr<Job>> jobs;
     uint64_t next_id = 1;
     uint64_t resident = 0;       // bytes of admitted jobs (arena)
     uint64_t in_system = 0;      // admitted and not yet done
+    Clock::time_point busy_since{};  // when in_system last became non-zero
     bool stop_admit = false, stop_loop = false, stop_read = false;
-    std::thread t_admit, t_loop, t_read;
+    std::thread t_admit, t_read;
 
     hs_server_stats st{};
     std::vector<std::pair<double, uint64_t>> step_ms;     // (launch device ms, jobs in it)
@@ -196,9 +208,18 @@ struct hs_server {
         }
     }
     bool fits(const Job* j) const { return resident + j->bytes + j->upload <= cfg.hbm_budget; }
+    // in_system changes (with `m` held); busy_s counts the time it is non-zero
+    void enter()
+    {
+        if (in_system++ == 0) busy_since = Clock::now();
+    }
+    void leave()
+    {
+        if (--in_system == 0) st.busy_s += seconds_since(busy_since);
+    }
     void fail_job(Job* j);
     void admit_loop();
-    void step_loop();
+    void step_loop(Loop& L);
     void read_loop();
 };
 
@@ -208,12 +229,13 @@ struct hs_server {
 void hs_server::fail_job(Job* j)
 {
     hsi::Groom g;
-    if (hsi::ctx_take(ctx, j->groom_id, &g)) hsi::groom_release(g);
+    if (j->loop && hsi::ctx_take(j->loop->ctx, j->groom_id, &g)) hsi::groom_release(g);
     hsi::groom_release(j->g);  // a job whose last step was enqueued holds its groom itself
+    if (j->loop) j->loop->points -= j->points;
     if (j->staging) arena.release(j->staging);
     j->staging = nullptr;
     resident -= j->bytes;
-    in_system -= 1;
+    leave();
     j->state = JobState::Failed;
 }
 
@@ -227,7 +249,7 @@ void hs_server::admit_loop()
         Job* j = queue.front();
         queue.pop_front();
         resident += j->bytes;
-        in_system += 1;
+        enter();
         st.admitted += 1;
         st.peak_resident_bytes = std::max(st.peak_resident_bytes, resident);
         j->state = JobState::Admitted;
@@ -244,7 +266,7 @@ void hs_server::admit_loop()
         if (rc == HS_ERR_OUT_OF_MEMORY || e == hipErrorOutOfMemory) {
             // the arena is fragmented: wait for a release unless nothing else holds memory
             resident -= j->bytes;
-            in_system -= 1;
+            leave();
             st.admitted -= 1;
             if (resident == 0) {
                 j->state = JobState::Failed;
@@ -259,18 +281,24 @@ void hs_server::admit_loop()
         }
         if (rc != HS_OK || e != hipSuccess) {
             resident -= j->bytes;
-            in_system -= 1;
+            leave();
             j->state = JobState::Failed;
             set_error(rc != HS_OK ? rc : HS_ERR_HIP, "hs_server admission: " + why);
             cv_done.notify_all();
             continue;
         }
-        ready.push_back(j);
-        cv_loop.notify_one();
+        // the loop with the fewest points takes the job
+        Loop* best = loops[0].get();
+        for (auto& L : loops)
+            if (L->points < best->points) best = L.get();
+        j->loop = best;
+        best->points += j->points;
+        best->ready.push_back(j);
+        cv_loop.notify_all();
     }
 }
 
-void hs_server::step_loop()
+void hs_server::step_loop(Loop& L)
 {
     (void)hipSetDevice(cfg.device);
     std::deque<Job*> active;  // resident jobs in round-robin order
@@ -278,9 +306,9 @@ void hs_server::step_loop()
     std::vector<hs_batch_item> items;
     std::vector<hs_collider> cols;
     std::vector<hs_step_params> params;
+    hs_ctx* ctx = L.ctx;
+    Flight* flight = L.flight;
     int head = 0, inflight = 0;
-    Clock::time_point busy_since{};
-    bool busy = false;
 
     // After a failed enqueue or launch: drain the stream, then fail every
     // resident job (those stepped in the failed launch included).
@@ -289,7 +317,7 @@ void hs_server::step_loop()
         std::lock_guard<std::mutex> lk(m);
         set_error(code, msg);
         for (int q = 0; q < inflight; ++q) {
-            Flight& f = flight[(head + q) % kLaunchDepth];
+            Flight& f = flight[(head + q) % depth];
             for (Job* j : f.last) fail_job(j);
             f.last.clear();
         }
@@ -313,7 +341,7 @@ void hs_server::step_loop()
             fail_all(HS_ERR_HIP, std::string("hs_server step: ") + hipGetErrorString(e));
             return false;
         }
-        head = (head + 1) % kLaunchDepth;
+        head = (head + 1) % depth;
         --inflight;
         std::lock_guard<std::mutex> lk(m);
         st.launches += 1;
@@ -335,23 +363,15 @@ void hs_server::step_loop()
         {
             std::unique_lock<std::mutex> lk(m);
             if (active.empty() && inflight == 0) {
-                if (busy) {
-                    st.busy_s += seconds_since(busy_since);
-                    busy = false;
-                }
-                cv_loop.wait(lk, [&] { return stop_loop || !ready.empty(); });
-                if (ready.empty()) break;  // stop_loop with nothing left
+                cv_loop.wait(lk, [&] { return stop_loop || !L.ready.empty(); });
+                if (L.ready.empty()) break;  // stop_loop with nothing left
             }
-            for (Job* j : ready) {
+            for (Job* j : L.ready) {
                 j->groom_id = hsi::ctx_adopt(ctx, std::move(j->g));
                 j->state = JobState::Active;
                 active.push_back(j);
             }
-            ready.clear();
-        }
-        if (!busy) {
-            busy = true;
-            busy_since = Clock::now();
+            L.ready.clear();
         }
         if (active.empty()) {
             retire();  // only launches on the stream are left
@@ -396,8 +416,8 @@ void hs_server::step_loop()
         }
         // the host work above overlapped the launches on the stream; now
         // make room for this one
-        if (inflight == kLaunchDepth && !retire()) continue;
-        Flight& f = flight[(head + inflight) % kLaunchDepth];
+        if (inflight == depth && !retire()) continue;
+        Flight& f = flight[(head + inflight) % depth];
         f.enq = Clock::now();
         int rc = HS_OK;
         hipError_t e = hipEventRecord(f.t0, ctx->stream);
@@ -422,6 +442,8 @@ void hs_server::step_loop()
             if (j->done_steps >= j->spec.steps) {
                 // out of the context now; its memory stays until read back
                 hsi::ctx_take(ctx, j->groom_id, &j->g);
+                L.points -= j->points;
+                j->loop = nullptr;
                 f.last.push_back(j);
             } else {
                 active.push_back(j);
@@ -455,7 +477,7 @@ void hs_server::read_loop()
         j->staging = nullptr;
         lk.lock();
         resident -= j->bytes;
-        in_system -= 1;
+        leave();
         if (e != hipSuccess) {
             set_error(HS_ERR_HIP, std::string("hs_server read-back: ") + hipGetErrorString(e));
             j->state = JobState::Failed;
@@ -500,39 +522,62 @@ void hs_keyframes_eval(const hs_keyframe* keys, uint32_t n, float t, float wind[
     }
 }
 
+// Releases what hs_server_create made (the threads are not running).
+static void server_free(hs_server* s)
+{
+    (void)hipSetDevice(s->cfg.device);
+    if (s->up) (void)hipStreamSynchronize(s->up);
+    if (s->down) (void)hipStreamSynchronize(s->down);
+    for (auto& L : s->loops) {
+        if (L->ctx) hs_destroy(L->ctx);
+        for (Flight& f : L->flight) {
+            if (f.t0) (void)hipEventDestroy(f.t0);
+            if (f.t1) (void)hipEventDestroy(f.t1);
+        }
+    }
+    if (s->up) (void)hipStreamDestroy(s->up);
+    if (s->down) (void)hipStreamDestroy(s->down);
+    delete s;
+}
+
 int hs_server_create(const hs_server_config* cfg, hs_server** out)
 {
     if (!cfg || !out) return HS_ERR_INVALID;
     *out = nullptr;
     if (cfg->hbm_budget == 0 || cfg->max_points_per_step == 0) return HS_ERR_INVALID;
-    std::unique_ptr<hs_server> s(new (std::nothrow) hs_server());
+    if (cfg->step_loops > HS_SERVER_MAX_STEP_LOOPS || cfg->launch_depth > 2) return HS_ERR_INVALID;
+    hs_server* s = new (std::nothrow) hs_server();
     if (!s) return HS_ERR_OUT_OF_MEMORY;
     s->cfg = *cfg;
-    int rc = hs_create(&s->ctx, cfg->device);
-    if (rc != HS_OK) return rc;
-    if (cfg->use_graphs) hs_set_batch_graphs(s->ctx, 1);
-    hipError_t e = s->arena.init(cfg->hbm_budget);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->up, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->down, hipStreamNonBlocking);
-    for (Flight& f : s->flight) {
-        if (e == hipSuccess) e = hipEventCreate(&f.t0);
-        if (e == hipSuccess) e = hipEventCreate(&f.t1);
-    }
-    if (e != hipSuccess) {
-        if (s->up) (void)hipStreamDestroy(s->up);
-        if (s->down) (void)hipStreamDestroy(s->down);
-        for (Flight& f : s->flight) {
-            if (f.t0) (void)hipEventDestroy(f.t0);
-            if (f.t1) (void)hipEventDestroy(f.t1);
+    s->depth = cfg->launch_depth ? (int)cfg->launch_depth : 1;
+    const uint32_t nloops = cfg->step_loops ? cfg->step_loops : 4;
+    int rc = HS_OK;
+    hipError_t e = hipSuccess;
+    for (uint32_t k = 0; k < nloops && rc == HS_OK && e == hipSuccess; ++k) {
+        s->loops.push_back(std::make_unique<Loop>());
+        Loop& L = *s->loops.back();
+        rc = hs_create(&L.ctx, cfg->device);
+        if (rc == HS_OK && cfg->use_graphs) rc = hs_set_batch_graphs(L.ctx, 1);
+        for (Flight& f : L.flight) {
+            if (rc == HS_OK && e == hipSuccess) e = hipEventCreate(&f.t0);
+            if (rc == HS_OK && e == hipSuccess) e = hipEventCreate(&f.t1);
         }
-        hs_destroy(s->ctx);
+    }
+    if (rc == HS_OK && e == hipSuccess) e = s->arena.init(cfg->hbm_budget);
+    if (rc == HS_OK && e == hipSuccess) e = hipStreamCreateWithFlags(&s->up, hipStreamNonBlocking);
+    if (rc == HS_OK && e == hipSuccess) e = hipStreamCreateWithFlags(&s->down, hipStreamNonBlocking);
+    if (rc != HS_OK || e != hipSuccess) {
+        server_free(s);
+        if (rc != HS_OK) return rc;
         return e == hipErrorOutOfMemory ? HS_ERR_OUT_OF_MEMORY : HS_ERR_HIP;
     }
-    hs_server* p = s.get();
-    p->t_admit = std::thread([p] { p->admit_loop(); });
-    p->t_loop = std::thread([p] { p->step_loop(); });
-    p->t_read = std::thread([p] { p->read_loop(); });
-    *out = s.release();
+    s->t_admit = std::thread([s] { s->admit_loop(); });
+    for (auto& L : s->loops) {
+        Loop* l = L.get();
+        l->t = std::thread([s, l] { s->step_loop(*l); });
+    }
+    s->t_read = std::thread([s] { s->read_loop(); });
+    *out = s;
     return HS_OK;
 }
 
@@ -643,7 +688,7 @@ void hs_server_destroy(hs_server* s)
     }
     s->cv_admit.notify_all();
     s->t_admit.join();
-    // admission has stopped: whatever it admitted is in `ready` or beyond
+    // admission has stopped: whatever it admitted is in a loop's `ready` or beyond
     {
         std::unique_lock<std::mutex> lk(s->m);
         s->cv_done.wait(lk, [&] { return s->in_system == 0; });
@@ -652,19 +697,9 @@ void hs_server_destroy(hs_server* s)
     }
     s->cv_loop.notify_all();
     s->cv_read.notify_all();
-    s->t_loop.join();
+    for (auto& L : s->loops) L->t.join();
     s->t_read.join();
-    (void)hipSetDevice(s->cfg.device);
-    (void)hipStreamSynchronize(s->up);
-    (void)hipStreamSynchronize(s->down);
-    hs_destroy(s->ctx);
-    (void)hipStreamDestroy(s->up);
-    (void)hipStreamDestroy(s->down);
-    for (Flight& f : s->flight) {
-        (void)hipEventDestroy(f.t0);
-        (void)hipEventDestroy(f.t1);
-    }
-    delete s;
+    server_free(s);
 }
 
 }  // extern "C"

@@ -65,7 +65,8 @@ def test_served_jobs_equal_alone(ctx):
     refs = [_alone(ctx, j, forcing=i % 3) for i, j in enumerate(jobs)]
     sizes = [hairsim.groom_bytes(j["aos"].size // (12 * j["N"]), j["N"], 16) + j["aos"].nbytes for j in jobs]
     budget = int(sum(sorted(sizes)[-3:]) * 1.05)  # about three jobs resident at once
-    with hairsim.Server(0, hbm_budget=budget, max_points_per_step=60000) as srv:
+    # one step loop: with several, each of the few resident jobs would have a loop to itself
+    with hairsim.Server(0, hbm_budget=budget, max_points_per_step=60000, step_loops=1, launch_depth=2) as srv:
         ids = []
         for i, j in enumerate(jobs):
             jid, _ = srv.submit(j["aos"], j["N"], j["steps"], DT, j["keys"], _params(forcing=i % 3),
@@ -89,16 +90,18 @@ def test_served_jobs_equal_alone(ctx):
     assert 0.0 < st["gpu_busy_s"] <= st["busy_s"] * 1.05
 
 
-def test_round_robin_launches_equal_alone(ctx):
+@pytest.mark.parametrize("loops,depth", [(1, 2), (3, 1), (2, 2)])
+def test_round_robin_launches_equal_alone(ctx, loops, depth):
     """Every job resident at once (a large HBM budget) but a point limit of
-    about three jobs per launch: launches take jobs in round-robin order, two
-    launches are queued at a time, and every job still ends bit-identical to
-    stepping it alone."""
+    about three jobs per launch: launches take jobs in round-robin order, up
+    to `depth` launches are queued per step loop, `loops` loops launch
+    concurrently on their own streams, and every job still ends bit-identical
+    to stepping it alone."""
     jobs = _job_set(91, 8)
     refs = [_alone(ctx, j, forcing=i % 3) for i, j in enumerate(jobs)]
     pts = [j["aos"].size // 12 for j in jobs]
     cap = int(sum(sorted(pts)[-3:]))
-    with hairsim.Server(0, hbm_budget=1 << 30, max_points_per_step=cap) as srv:
+    with hairsim.Server(0, hbm_budget=1 << 30, max_points_per_step=cap, step_loops=loops, launch_depth=depth) as srv:
         ids = [srv.submit(j["aos"], j["N"], j["steps"], DT, j["keys"], _params(forcing=i % 3), seg_length=j["seg"],
                           wind_noise=j["noise"], **GRID)[0] for i, j in enumerate(jobs)]
         for i, jid in enumerate(ids):
