@@ -12,12 +12,14 @@
 // groom exactly as the single-groom ones do, and a job's steps run in order
 // on one stream.
 //
-// Threads: admission (FIFO; uploads on its own stream), the step loop (owns
-// the context and its stream) and completion (reads finished jobs back on a
-// third stream while later launches run).  The step loop keeps up to
-// kLaunchDepth launches queued on the stream: it builds launch k+1 on the
-// host while launch k runs and only then waits for launch k-1, so the device
-// does not idle between launches.
+// Threads: admission (FIFO; uploads on its own stream; each admitted job goes
+// to the step loop holding the fewest points), the step loops (each owns a
+// context and its stream; a job stays in its loop) and completion (reads
+// finished jobs back on another stream while later launches run).  A loop
+// builds its next launch on the host while its current one runs and keeps at
+// most `depth` launches queued on its stream; the loops' launches run
+// concurrently, so one launch's last round of workgroups and a loop's gaps
+// between launches are filled by the others' work.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
