@@ -437,7 +437,8 @@ def bench_served(args, hairsim, scenes):
     prm = hairsim.step_params(damping=0.9, sphere_radius=0.3, friction=0.05, num_colliders=1, hair_hair=True)
     budget = 200 << 30
     srv = hairsim.Server(0, hbm_budget=budget, max_points_per_step=args.max_points, use_graphs=False,
-                         step_loops=args.step_loops, launch_depth=args.launch_depth)
+                         step_loops=args.step_loops, launch_depth=args.launch_depth, tile=args.tile)
+    srv_tile = args.tile or (2048, 2048, 0, 256, 8)
     n_jobs = int(args.rate * args.seconds * 1.5) + 16
     gaps = rng.exponential(1.0 / args.rate, n_jobs)
     arrivals = np.cumsum(gaps)
@@ -522,7 +523,7 @@ def bench_served(args, hairsim, scenes):
         "config": {"workload": f"served: Poisson {args.rate:g} jobs/s for {args.seconds:g} s, S log-uniform "
                                f"[256, 32768], N in {{16,32,64}}, 60-600 steps, keyframes every 0.25 s, 32^3 grids, "
                                f"200 GB budget, <= {args.max_points} points per launch, round-robin, {args.step_loops} step loop(s) "
-                               f"x {args.launch_depth} queued launch(es)",
+                               f"x {args.launch_depth} queued launch(es), tile {srv_tile[3]} x {srv_tile[4]}",
                    "jobs_submitted": st["submitted"], "jobs_completed": st["completed"],
                    "jobs_cancelled_unadmitted": cancelled},
         "served": {"p50_step_ms": st["p50_step_ms"], "p99_step_ms": st["p99_step_ms"],
@@ -598,7 +599,7 @@ def main():
                     choices=["plain-100k", "large-1m", "plain-1k", "batched-256", "served", "export-100k"])
     ap.add_argument("--rate", type=float, default=2000.0, help="served: job arrivals per second")
     ap.add_argument("--seconds", type=float, default=30.0, help="served: arrival window")
-    ap.add_argument("--max-points", type=int, default=7_000_000, help="served: points per batched launch")
+    ap.add_argument("--max-points", type=int, default=8_000_000, help="served: points per batched launch")
     ap.add_argument("--step-loops", type=int, default=4, help="served: step loops (contexts with their own stream)")
     ap.add_argument("--launch-depth", type=int, default=1, help="served: launches queued per step loop")
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -432,6 +432,10 @@ typedef struct hs_server_config {
                                   /* stream; their launches run concurrently    */
                                   /* (0: 4, at most HS_SERVER_MAX_STEP_LOOPS)    */
     uint32_t launch_depth;        /* launches queued per loop, 1 or 2 (0: 1)    */
+    hs_tile tile;                 /* step-kernel tile of every loop (all zero:  */
+                                  /* 256 strands x 8 points, 2048-cell windows, */
+                                  /* XCD order 0, the measured best for served  */
+                                  /* job mixes)                                 */
 } hs_server_config;
 
 typedef struct hs_server_stats {

@@ -552,6 +552,9 @@ int hs_server_create(const hs_server_config* cfg, hs_server** out)
     if (!s) return HS_ERR_OUT_OF_MEMORY;
     s->cfg = *cfg;
     s->depth = cfg->launch_depth ? (int)cfg->launch_depth : 1;
+    const hs_tile zero{};
+    const bool given = std::memcmp(&cfg->tile, &zero, sizeof(zero)) != 0;
+    const hs_tile tile = given ? cfg->tile : hs_tile{2048, 2048, 0, 256, 8};
     const uint32_t nloops = cfg->step_loops ? cfg->step_loops : 4;
     int rc = HS_OK;
     hipError_t e = hipSuccess;
@@ -560,6 +563,7 @@ int hs_server_create(const hs_server_config* cfg, hs_server** out)
         Loop& L = *s->loops.back();
         rc = hs_create(&L.ctx, cfg->device);
         if (rc == HS_OK && cfg->use_graphs) rc = hs_set_batch_graphs(L.ctx, 1);
+        if (rc == HS_OK) rc = hs_set_tile(L.ctx, &tile);
         for (Flight& f : L.flight) {
             if (rc == HS_OK && e == hipSuccess) e = hipEventCreate(&f.t0);
             if (rc == HS_OK && e == hipSuccess) e = hipEventCreate(&f.t1);

@@ -142,7 +142,7 @@ class HsJob(C.Structure):
 
 class HsServerConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("hbm_budget", C.c_uint64), ("max_points_per_step", C.c_uint64),
-                ("use_graphs", C.c_int), ("step_loops", C.c_uint32), ("launch_depth", C.c_uint32)]
+                ("use_graphs", C.c_int), ("step_loops", C.c_uint32), ("launch_depth", C.c_uint32), ("tile", HsTile)]
 
 
 class HsServerStats(C.Structure):
@@ -691,8 +691,11 @@ class Server:
     """Continuous-batching job server (hs_server_*): submit jobs, wait for their results."""
 
     def __init__(self, device=0, hbm_budget=1 << 30, max_points_per_step=1 << 24, use_graphs=False, step_loops=0,
-                 launch_depth=0):
-        cfg = HsServerConfig(device, hbm_budget, max_points_per_step, 1 if use_graphs else 0, step_loops, launch_depth)
+                 launch_depth=0, tile=None):
+        """tile: (gather_cells, splat_cells, xcd_order, strands, points) of every step loop, or None for the
+        server's default (256 x 8)."""
+        cfg = HsServerConfig(device, hbm_budget, max_points_per_step, 1 if use_graphs else 0, step_loops, launch_depth,
+                             HsTile(*tile) if tile is not None else HsTile())
         self._h = C.c_void_p()
         rc = lib().hs_server_create(C.byref(cfg), C.byref(self._h))
         if rc != HS_OK:
