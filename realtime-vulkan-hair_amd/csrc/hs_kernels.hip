@@ -96,10 +96,7 @@ __device__ __forceinline__ f3 ld_f3(const float4* p)
 // No reference counterpart.  Evaluated by the mesh pass (k_mesh_force /
 // k_mesh_force_batch) for every stored point before the step kernel, which
 // reads the result.
-#ifndef HS_MESH_PASS_ROUND
-#define HS_MESH_PASS_ROUND 2
-#endif
-constexpr uint32_t kMeshRound = HS_MESH_PASS_ROUND;  // mesh candidates loaded per round
+constexpr uint32_t kMeshRound = 2;  // mesh candidates loaded per round
 
 __device__ __forceinline__ float4 mesh_contact(const MeshArgs& M, float px, float py, float pz, float k)
 {

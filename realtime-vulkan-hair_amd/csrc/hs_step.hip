@@ -44,16 +44,8 @@ constexpr int kWaves = kW / 64;
 
 // Minimum over the wavefront (every lane active): DPP steps within each row
 // of 16 lanes (VALU moves, no LDS traffic), then the four row results.
-#ifndef HS_DPP_MIN
-#define HS_DPP_MIN 1
-#endif
 __device__ __forceinline__ int wave_min(int v)
 {
-#if !HS_DPP_MIN
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v = min(v, __shfl_xor(v, m, 64));
-    return v;
-#endif
     v = min(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
     v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
     v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false));  // row_half_mirror
