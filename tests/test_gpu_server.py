@@ -176,3 +176,22 @@ def test_bad_job_rejected_at_submit_others_complete(ctx):
         assert code == 0
         st = srv.stats()
     assert st["completed"] == len(jobs) and st["submitted"] == len(jobs)
+
+
+def test_server_tile_and_loops_config(ctx):
+    """An explicit tile for the step loops (here the library's 128 x 16) gives
+    the same bits as the server's default tile and as stepping alone; a tile
+    shape that is not built, a launch depth above 2 or too many loops are
+    refused at creation."""
+    jobs = _job_set(13, 3)
+    refs = [_alone(ctx, j) for j in jobs]
+    for kw in (dict(tile=(2048, 1024, 1, 128, 16), step_loops=2, launch_depth=2), dict(step_loops=1, launch_depth=1)):
+        with hairsim.Server(0, hbm_budget=1 << 30, max_points_per_step=1 << 22, **kw) as srv:
+            ids = [srv.submit(j["aos"], j["N"], j["steps"], DT, j["keys"], _params(), seg_length=j["seg"],
+                              wind_noise=j["noise"], **GRID)[0] for j in jobs]
+            for i, jid in enumerate(ids):
+                assert_bitwise(srv.wait(jid, timeout=60.0), refs[i], f"job {i} {kw}", jobs[i]["N"])
+    for bad in (dict(tile=(2048, 2048, 1, 64, 16)), dict(launch_depth=3), dict(step_loops=9)):
+        with pytest.raises(hairsim.HairSimError) as e:
+            hairsim.Server(0, hbm_budget=1 << 30, max_points_per_step=1 << 22, **bad)
+        assert e.value.code == hairsim.HS_ERR_INVALID, bad
