@@ -60,10 +60,11 @@ __device__ __forceinline__ int32_t sat_i32(float f)
 __device__ __forceinline__ void cell_range(float q, int G, int& lo, int& hi)
 {
     float f = floorf(q);
-    if (!(f == f)) f = 0.0f;                // NaN -> int() == 0
-    if (f < -2.0f) { lo = 0; hi = -1; return; }
-    if (f > (float)G) { lo = G; hi = G - 1; return; }
-    int fi = (int)f;
+    if (!(f == f)) f = 0.0f;  // NaN -> int() == 0
+    // Clamped to [-2, G] without branches: below -2 the range is (0, -1) and
+    // above G it is (G, G - 1), both empty, exactly as for f = -2 and f = G.
+    f = fminf(fmaxf(f, -2.0f), (float)G);
+    const int fi = (int)f;
     lo = fi > 0 ? fi : 0;
     hi = (fi + 1) < (G - 1) ? (fi + 1) : (G - 1);
 }
