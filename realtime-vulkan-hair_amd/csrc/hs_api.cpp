@@ -823,33 +823,20 @@ int hs_mesh_create(hs_ctx* c, const float* v, uint32_t nv, const uint32_t* t, ui
                     "hs_mesh_create: mesh is not closed (or an index is out of range, or a vertex is not finite)");
     HS_TRY(c, hipSetDevice(c->device));
     const uint64_t tri_bytes = round_up(b.tris.size() * 4, 256);
-    const uint64_t sph_bytes = round_up(b.spheres.size() * 4, 256);
     const uint64_t start_bytes = round_up(b.cell_start.size() * 4, 256);
-    const uint64_t ids_bytes = round_up(std::max<size_t>(b.cell_tris.size(), 1) * 4, 256);
-    const uint64_t csph_bytes = round_up(std::max<size_t>(b.cell_spheres.size(), 4) * 4, 256);
-    const uint64_t key_bytes = round_up(std::max<size_t>(b.cell_keys.size(), 1) * 4, 256);
+    const uint64_t rec_bytes = round_up(std::max<size_t>(b.cell_recs.size(), hsmesh::kRecFloats) * 4, 256);
     hsi::Mesh m;
-    m.bytes = tri_bytes + sph_bytes + start_bytes + ids_bytes + csph_bytes + key_bytes;
+    m.bytes = tri_bytes + start_bytes + rec_bytes;
     m.ntris = nt;
     HS_TRY(c, hipMalloc(&m.block, m.bytes));
     uint8_t* base = static_cast<uint8_t*>(m.block);
-    uint8_t* sph = base + tri_bytes;
-    uint8_t* starts = sph + sph_bytes;
-    uint8_t* ids = starts + start_bytes;
-    uint8_t* csph = ids + ids_bytes;
-    uint8_t* keys = csph + csph_bytes;
+    uint8_t* starts = base + tri_bytes;
+    uint8_t* recs = starts + start_bytes;  // 256-aligned: each 64 B record is one aligned line
     hipError_t e = hipMemcpyAsync(base, b.tris.data(), b.tris.size() * 4, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(sph, b.spheres.data(), b.spheres.size() * 4, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess)
         e = hipMemcpyAsync(starts, b.cell_start.data(), b.cell_start.size() * 4, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess && !b.cell_tris.empty())
-        e = hipMemcpyAsync(ids, b.cell_tris.data(), b.cell_tris.size() * 4, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess && !b.cell_spheres.empty())
-        e = hipMemcpyAsync(csph, b.cell_spheres.data(), b.cell_spheres.size() * 4, hipMemcpyHostToDevice,
-                           c->stream);
-    if (e == hipSuccess && !b.cell_keys.empty())
-        e = hipMemcpyAsync(keys, b.cell_keys.data(), b.cell_keys.size() * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && !b.cell_recs.empty())
+        e = hipMemcpyAsync(recs, b.cell_recs.data(), b.cell_recs.size() * 4, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         (void)hipFree(m.block);
@@ -857,11 +844,8 @@ int hs_mesh_create(hs_ctx* c, const float* v, uint32_t nv, const uint32_t* t, ui
     }
     hs::MeshArgs& a = m.args;
     a.tris = reinterpret_cast<const float4*>(base);
-    a.spheres = reinterpret_cast<const float4*>(sph);
     a.cell_start = reinterpret_cast<const uint32_t*>(starts);
-    a.cell_tris = reinterpret_cast<const uint32_t*>(ids);
-    a.cell_spheres = reinterpret_cast<const float4*>(csph);
-    a.cell_keys = reinterpret_cast<const float*>(keys);
+    a.cell_recs = reinterpret_cast<const float4*>(recs);
     a.half_diag = b.half_diag;
     a.ox = b.origin[0];
     a.oy = b.origin[1];

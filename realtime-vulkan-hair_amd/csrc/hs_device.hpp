@@ -401,11 +401,8 @@ __device__ __forceinline__ MeshArgs global_mesh(const MeshArgs& in)
 {
     MeshArgs m = in;
     m.tris = global_ptr(in.tris);
-    m.spheres = global_ptr(in.spheres);
     m.cell_start = global_ptr(in.cell_start);
-    m.cell_tris = global_ptr(in.cell_tris);
-    m.cell_spheres = global_ptr(in.cell_spheres);
-    m.cell_keys = global_ptr(in.cell_keys);
+    m.cell_recs = global_ptr(in.cell_recs);
     return m;
 }
 

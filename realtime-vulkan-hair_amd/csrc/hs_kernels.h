@@ -66,14 +66,14 @@ struct GridWork {
 // Everything one step launch needs, passed by value as the kernel argument.
 // Triangle-mesh collider: per-triangle records of 10 float4 (a, b, c, face
 // normal, edge pseudo-normals ab bc ca, vertex pseudo-normals a b c) and a
-// uniform grid of ascending triangle-id lists.
+// uniform grid of candidate lists.
 struct MeshArgs {
     const float4* tris;
-    const float4* spheres;       // per triangle: centroid, bounding radius
     const uint32_t* cell_start;  // [cells + 1]
-    const uint32_t* cell_tris;
-    const float4* cell_spheres;  // bounding sphere of each cell_tris entry
-    const float* cell_keys;      // lower bound of the cell-triangle distance (ascending per cell)
+    // [entries][4]: (a, key), (b, triangle id bits), (c, bounding radius),
+    // (bounding centre, 0); key = a lower bound of the distance from the
+    // cell's centre to the triangle, ascending within a cell
+    const float4* cell_recs;
     float half_diag;             // half the cell diagonal
     float ox, oy, oz;            // grid origin
     float icx, icy, icz;         // 1 / cell size

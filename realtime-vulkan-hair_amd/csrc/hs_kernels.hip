@@ -87,6 +87,15 @@ __device__ __forceinline__ f3 ld_f3(const float4* p)
     return mk(v.x, v.y, v.z);
 }
 
+// Marks a loaded value as used here (an empty asm statement that reads and
+// may rewrite its registers; it emits no instruction).  Placed after a round's
+// loads, it keeps the compiler from sinking a load into the branch that uses
+// it, so all of them are issued before the first wait.
+__device__ __forceinline__ void loaded_here(float4& v)
+{
+    asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w));
+}
+
 // Mesh contact: the mesh point q nearest to x among the triangles listed for
 // x's grid cell (every triangle within the margin of the cell is listed, so
 // within the margin this is the nearest over the whole mesh; lowest id wins
@@ -103,9 +112,11 @@ __device__ __forceinline__ float4 mesh_contact(const MeshArgs& M, float px, floa
     const f3 x = mk(px, py, pz);
     const float4 none = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const float gx = (x.x - M.ox) * M.icx, gy = (x.y - M.oy) * M.icy, gz = (x.z - M.oz) * M.icz;
-    // outside the grid (or NaN): farther than the margin from every triangle
-    if (!(gx >= 0.0f && gx < (float)M.dx && gy >= 0.0f && gy < (float)M.dy && gz >= 0.0f && gz < (float)M.dz))
-        return none;
+    // outside the grid (or NaN): farther than the margin from every triangle.
+    // Non-short-circuit '&', so the three coordinate loads are issued together.
+    const bool inside = (gx >= 0.0f) & (gx < (float)M.dx) & (gy >= 0.0f) & (gy < (float)M.dy) & (gz >= 0.0f) &
+                        (gz < (float)M.dz);
+    if (!inside) return none;
     const uint32_t cx = min((uint32_t)gx, M.dx - 1), cy = min((uint32_t)gy, M.dy - 1),
                    cz = min((uint32_t)gz, M.dz - 1);
     const uint32_t cell = (cz * M.dy + cy) * M.dx + cx;
@@ -125,44 +136,51 @@ __device__ __forceinline__ float4 mesh_contact(const MeshArgs& M, float px, floa
     int best_feat = 0;
     uint32_t best_t = 0xffffffffu;
     f3 best_q = mk(0.0f, 0.0f, 0.0f);
-    // candidates kMeshRound at a time: their ids, spheres and keys are contiguous in
-    // the cell's list, so the loads are in flight together.  The list is
-    // sorted by `key` (the distance from the cell's centre to the triangle's
-    // bounding sphere, rounded down); key - |x - centre| bounds the distance
-    // from below, so once that passes `reach` no later entry can win.  The
-    // nearest of equal distances is the lowest id (the order of the list
-    // does not matter).
+    // candidates kMeshRound at a time: a candidate is one 64 B record holding
+    // its key, bounding sphere, id and vertices, so every load of a round is
+    // in flight together and none waits on an id.  The list is sorted by `key`
+    // (the distance from the cell's centre to the triangle's bounding sphere,
+    // rounded down); key - |x - centre| bounds the distance from below, so
+    // once that passes `reach` no later entry can win.  The nearest of equal
+    // distances is the lowest id (the order of the list does not matter).
     bool more = true;
     for (uint32_t i = i0; i < i1 && more; i += kMeshRound) {
-        float4 sp[kMeshRound];
-        uint32_t tt[kMeshRound];
-        float kk[kMeshRound];
+        float4 ra[kMeshRound], rb[kMeshRound], rc[kMeshRound], rs[kMeshRound];
 #pragma unroll
         for (uint32_t u = 0; u < kMeshRound; ++u) {
-            const uint32_t e = min(i + u, i1 - 1);
-            sp[u] = M.cell_spheres[e];
-            tt[u] = M.cell_tris[e];
-            kk[u] = M.cell_keys[e];
+            const float4* r = M.cell_recs + 4 * (size_t)min(i + u, i1 - 1);
+            ra[u] = r[0];
+            rb[u] = r[1];
+            rc[u] = r[2];
+            rs[u] = r[3];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kMeshRound; ++u) {
+            loaded_here(ra[u]);
+            loaded_here(rb[u]);
+            loaded_here(rc[u]);
+            loaded_here(rs[u]);
         }
 #pragma unroll
         for (uint32_t u = 0; u < kMeshRound; ++u) {
             if (i + u >= i1 || !more) break;
-            if (kk[u] - dxc > reach) {
+            if (ra[u].w - dxc > reach) {
                 more = false;
                 break;
             }
-            const f3 dc = sub(x, mk(sp[u].x, sp[u].y, sp[u].z));
-            const float lim = sp[u].w + reach;
+            const f3 dc = sub(x, mk(rs[u].x, rs[u].y, rs[u].z));
+            const float lim = rc[u].w + reach;
             if (dot(dc, dc) > lim * lim * 1.0001f + 1e-12f) continue;
-            const float4* r = M.tris + (size_t)tt[u] * 10;
+            const uint32_t t = __float_as_uint(rb[u].w);
             int feat;
-            const f3 q = closest_on_tri(x, ld_f3(r), ld_f3(r + 1), ld_f3(r + 2), &feat);
+            const f3 q = closest_on_tri(x, mk(ra[u].x, ra[u].y, ra[u].z), mk(rb[u].x, rb[u].y, rb[u].z),
+                                        mk(rc[u].x, rc[u].y, rc[u].z), &feat);
             const f3 dq = sub(x, q);
             const float d2 = dot(dq, dq);
-            if (d2 < best || (d2 == best && best_t != 0xffffffffu && tt[u] < best_t)) {
+            if (d2 < best || (d2 == best && best_t != 0xffffffffu && t < best_t)) {
                 best = d2;
                 reach = sqrtf(best) + slack;
-                best_t = tt[u];
+                best_t = t;
                 best_feat = feat;
                 best_q = q;
             }

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 #include <map>
 #include <utility>
 #include <vector>
@@ -21,13 +22,18 @@ namespace hsmesh {
 // Baerentzen & Aanaes 2005).  10 float4 (w unused) = 160 B.
 constexpr int kTriFloats = 40;
 
+// One entry of a cell's candidate list, 64 B: (a, key), (b, triangle id as
+// float bits), (c, bounding radius), (bounding centre, 0).  The vertices are
+// copies of the triangle record's, so one candidate is one 64 B line and the
+// kernel's loads of a round do not wait on an id.
+constexpr int kRecFloats = 16;
+
 struct Built {
     std::vector<float> tris;          // [T][kTriFloats]
     std::vector<float> spheres;       // [T][4]: centroid, radius (rounded outward) of each triangle
     std::vector<uint32_t> cell_start; // [cells + 1]
-    std::vector<uint32_t> cell_tris;  // triangle ids per cell, nearest bounding sphere first
-    std::vector<float> cell_spheres;  // [entries][4]: the bounding sphere of each listed triangle
-    std::vector<float> cell_keys;     // [entries]: |cell centre - sphere centre| - radius, rounded down
+    std::vector<float> cell_recs;     // [entries][kRecFloats], per cell nearest bounding sphere first;
+                                      // key = |cell centre - sphere centre| - radius, rounded down
     float half_diag = 0.0f;           // half the cell diagonal, rounded up
     float origin[3] = {0, 0, 0};
     float cell[3] = {1, 1, 1};
@@ -213,8 +219,8 @@ inline bool build(const float* v, uint32_t nv, const uint32_t* t, uint32_t nt, c
     B.half_diag = (float)(hd * (1.0 + 1e-5) + 1e-6);
     std::vector<std::pair<float, uint32_t>> keyed;
     B.cell_start.assign(ncell + 1, 0);
-    B.cell_tris.clear();
-    B.cell_keys.clear();
+    B.cell_recs.clear();
+    uint32_t entries = 0;
     for (uint64_t cidx = 0; cidx < ncell; ++cidx) {
         const uint64_t x = cidx % B.dim[0], y = (cidx / B.dim[0]) % B.dim[1], z = cidx / ((uint64_t)B.dim[0] * B.dim[1]);
         const D3 cc{B.origin[0] + (x + 0.5) * B.cell[0], B.origin[1] + (y + 0.5) * B.cell[1],
@@ -229,16 +235,17 @@ inline bool build(const float* v, uint32_t nv, const uint32_t* t, uint32_t nt, c
         }
         std::sort(keyed.begin(), keyed.end());
         for (auto& kf : keyed) {
-            B.cell_tris.push_back(kf.second);
-            B.cell_keys.push_back(kf.first);
+            const float* r = &B.tris[(size_t)kf.second * kTriFloats];
+            const float* sp = &B.spheres[(size_t)kf.second * 4];
+            float id;
+            std::memcpy(&id, &kf.second, 4);
+            const float rec[kRecFloats] = {r[0], r[1], r[2],  kf.first, r[4],  r[5],  r[6],  id,
+                                           r[8], r[9], r[10], sp[3],    sp[0], sp[1], sp[2], 0.0f};
+            B.cell_recs.insert(B.cell_recs.end(), rec, rec + kRecFloats);
         }
-        B.cell_start[cidx + 1] = (uint32_t)B.cell_tris.size();
+        entries += (uint32_t)keyed.size();
+        B.cell_start[cidx + 1] = entries;
     }
-    // the spheres again, in list order: a cell's candidates are read as one
-    // contiguous stream, without waiting for each id first
-    B.cell_spheres.resize(B.cell_tris.size() * 4);
-    for (size_t e = 0; e < B.cell_tris.size(); ++e)
-        for (int k = 0; k < 4; ++k) B.cell_spheres[4 * e + k] = B.spheres[(size_t)B.cell_tris[e] * 4 + k];
     return true;
 }
 
