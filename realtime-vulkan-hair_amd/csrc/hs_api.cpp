@@ -831,7 +831,7 @@ int hs_mesh_create(hs_ctx* c, const float* v, uint32_t nv, const uint32_t* t, ui
     HS_TRY(c, hipMalloc(&m.block, m.bytes));
     uint8_t* base = static_cast<uint8_t*>(m.block);
     uint8_t* starts = base + tri_bytes;
-    uint8_t* recs = starts + start_bytes;  // 256-aligned: each 64 B record is one aligned line
+    uint8_t* recs = starts + start_bytes;
     hipError_t e = hipMemcpyAsync(base, b.tris.data(), b.tris.size() * 4, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(starts, b.cell_start.data(), b.cell_start.size() * 4, hipMemcpyHostToDevice, c->stream);

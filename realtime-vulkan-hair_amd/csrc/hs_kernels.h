@@ -70,9 +70,10 @@ struct GridWork {
 struct MeshArgs {
     const float4* tris;
     const uint32_t* cell_start;  // [cells + 1]
-    // [entries][4]: (a, key), (b, triangle id bits), (c, bounding radius),
-    // (bounding centre, 0); key = a lower bound of the distance from the
-    // cell's centre to the triangle, ascending within a cell
+    // [entries][3]: (a, key), (b, triangle id bits), (c, bounding radius);
+    // key = a lower bound of the distance from the cell's centre to the
+    // triangle, ascending within a cell; the bounding centre is
+    // ((a + b) + c) * (1/3) in binary32
     const float4* cell_recs;
     float half_diag;             // half the cell diagonal
     float ox, oy, oz;            // grid origin

@@ -136,30 +136,29 @@ __device__ __forceinline__ float4 mesh_contact(const MeshArgs& M, float px, floa
     int best_feat = 0;
     uint32_t best_t = 0xffffffffu;
     f3 best_q = mk(0.0f, 0.0f, 0.0f);
-    // candidates kMeshRound at a time: a candidate is one 64 B record holding
-    // its key, bounding sphere, id and vertices, so every load of a round is
-    // in flight together and none waits on an id.  The list is sorted by `key`
+    // candidates kMeshRound at a time: a candidate is one 48 B record holding
+    // its key, bounding radius, id and vertices, so every load of a round is
+    // in flight together and none waits on an id.  The bounding centre is
+    // formed from the vertices, as the host formed it for the radius.  The list is sorted by `key`
     // (the distance from the cell's centre to the triangle's bounding sphere,
     // rounded down); key - |x - centre| bounds the distance from below, so
     // once that passes `reach` no later entry can win.  The nearest of equal
     // distances is the lowest id (the order of the list does not matter).
     bool more = true;
     for (uint32_t i = i0; i < i1 && more; i += kMeshRound) {
-        float4 ra[kMeshRound], rb[kMeshRound], rc[kMeshRound], rs[kMeshRound];
+        float4 ra[kMeshRound], rb[kMeshRound], rc[kMeshRound];
 #pragma unroll
         for (uint32_t u = 0; u < kMeshRound; ++u) {
-            const float4* r = M.cell_recs + 4 * (size_t)min(i + u, i1 - 1);
+            const float4* r = M.cell_recs + 3 * (size_t)min(i + u, i1 - 1);
             ra[u] = r[0];
             rb[u] = r[1];
             rc[u] = r[2];
-            rs[u] = r[3];
         }
 #pragma unroll
         for (uint32_t u = 0; u < kMeshRound; ++u) {
             loaded_here(ra[u]);
             loaded_here(rb[u]);
             loaded_here(rc[u]);
-            loaded_here(rs[u]);
         }
 #pragma unroll
         for (uint32_t u = 0; u < kMeshRound; ++u) {
@@ -168,7 +167,10 @@ __device__ __forceinline__ float4 mesh_contact(const MeshArgs& M, float px, floa
                 more = false;
                 break;
             }
-            const f3 dc = sub(x, mk(rs[u].x, rs[u].y, rs[u].z));
+            const float third = 1.0f / 3.0f;
+            const f3 cen = mk(((ra[u].x + rb[u].x) + rc[u].x) * third, ((ra[u].y + rb[u].y) + rc[u].y) * third,
+                              ((ra[u].z + rb[u].z) + rc[u].z) * third);
+            const f3 dc = sub(x, cen);
             const float lim = rc[u].w + reach;
             if (dot(dc, dc) > lim * lim * 1.0001f + 1e-12f) continue;
             const uint32_t t = __float_as_uint(rb[u].w);

@@ -22,11 +22,22 @@ namespace hsmesh {
 // Baerentzen & Aanaes 2005).  10 float4 (w unused) = 160 B.
 constexpr int kTriFloats = 40;
 
-// One entry of a cell's candidate list, 64 B: (a, key), (b, triangle id as
-// float bits), (c, bounding radius), (bounding centre, 0).  The vertices are
-// copies of the triangle record's, so one candidate is one 64 B line and the
-// kernel's loads of a round do not wait on an id.
-constexpr int kRecFloats = 16;
+// One entry of a cell's candidate list, 48 B: (a, key), (b, triangle id as
+// float bits), (c, bounding radius).  The vertices are copies of the triangle
+// record's, so the kernel's loads of a round do not wait on an id; the
+// bounding sphere's centre is not stored: the kernel forms it from the
+// vertices with the same binary32 operations as sphere_centre below.
+constexpr int kRecFloats = 12;
+
+// Centre of a triangle's bounding sphere: ((a + b) + c) * (1/3) per axis in
+// binary32, the expression the mesh kernel evaluates (mesh_contact), so host
+// and device hold the same bits.
+inline float sphere_centre(float a, float b, float c)
+{
+    const float third = 1.0f / 3.0f;
+    const float s = a + b;
+    return (s + c) * third;
+}
 
 struct Built {
     std::vector<float> tris;          // [T][kTriFloats]
@@ -116,8 +127,8 @@ inline bool build(const float* v, uint32_t nv, const uint32_t* t, uint32_t nt, c
     for (uint32_t f = 0; f < nt; ++f) {
         const float* r = &B.tris[(size_t)f * kTriFloats];
         const D3 a{r[0], r[1], r[2]}, b{r[4], r[5], r[6]}, c{r[8], r[9], r[10]};
-        const D3 cen = mul(add(add(a, b), c), 1.0 / 3.0);
-        const float cf[3] = {(float)cen.x, (float)cen.y, (float)cen.z};
+        const float cf[3] = {sphere_centre(r[0], r[4], r[8]), sphere_centre(r[1], r[5], r[9]),
+                             sphere_centre(r[2], r[6], r[10])};
         const D3 cq{cf[0], cf[1], cf[2]};
         const double rad = std::max({std::sqrt(dot(sub(a, cq), sub(a, cq))), std::sqrt(dot(sub(b, cq), sub(b, cq))),
                                      std::sqrt(dot(sub(c, cq), sub(c, cq)))});
@@ -239,8 +250,7 @@ inline bool build(const float* v, uint32_t nv, const uint32_t* t, uint32_t nt, c
             const float* sp = &B.spheres[(size_t)kf.second * 4];
             float id;
             std::memcpy(&id, &kf.second, 4);
-            const float rec[kRecFloats] = {r[0], r[1], r[2],  kf.first, r[4],  r[5],  r[6],  id,
-                                           r[8], r[9], r[10], sp[3],    sp[0], sp[1], sp[2], 0.0f};
+            const float rec[kRecFloats] = {r[0], r[1], r[2], kf.first, r[4], r[5], r[6], id, r[8], r[9], r[10], sp[3]};
             B.cell_recs.insert(B.cell_recs.end(), rec, rec + kRecFloats);
         }
         entries += (uint32_t)keyed.size();
