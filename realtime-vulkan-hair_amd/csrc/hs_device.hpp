@@ -43,11 +43,13 @@ __device__ __forceinline__ Rng point_range(f3 q, int G)
     return r;
 }
 
-__device__ __forceinline__ bool rng_empty(const Rng& r) { return r.x0 > r.x1 || r.y0 > r.y1 || r.z0 > r.z1; }
+// The range tests combine their compares with non-short-circuit '|' and '&':
+// plain mask arithmetic instead of a branch per compare.
+__device__ __forceinline__ bool rng_empty(const Rng& r) { return (r.x0 > r.x1) | (r.y0 > r.y1) | (r.z0 > r.z1); }
 
 __device__ __forceinline__ bool rng_full(const Rng& r)
 {
-    return r.x1 == r.x0 + 1 && r.y1 == r.y0 + 1 && r.z1 == r.z0 + 1;
+    return (r.x1 == r.x0 + 1) & (r.y1 == r.y0 + 1) & (r.z1 == r.z0 + 1);
 }
 
 // A full 2x2x2 block packed in 31 bits (base voxel, 10 bits per axis, G <= 1024);
@@ -79,8 +81,8 @@ struct Win {
 
 __device__ __forceinline__ bool in_win(const Rng& r, const Win& w)
 {
-    return r.x0 >= w.x0 && r.x1 < w.x0 + w.ex && r.y0 >= w.y0 && r.y1 < w.y0 + w.ey && r.z0 >= w.z0 &&
-           r.z1 < w.z0 + w.ez;
+    return (r.x0 >= w.x0) & (r.x1 < w.x0 + w.ex) & (r.y0 >= w.y0) & (r.y1 < w.y0 + w.ey) & (r.z0 >= w.z0) &
+           (r.z1 < w.z0 + w.ez);
 }
 
 __device__ __forceinline__ int win_index(const Win& w, int a, int b, int c)

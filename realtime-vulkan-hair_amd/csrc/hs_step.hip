@@ -492,7 +492,7 @@ __device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
         for (uint32_t t = 0; t < kL; ++t) {
             X[t] = mk(0.0f, 0.0f, 0.0f);
             U[t] = mk(0.0f, 0.0f, 0.0f);
-            if (active && t < cnt) {
+            if (active & (t < cnt)) {
                 X[t] = ld3(posb, b0 + t * bS, P4, P8);
                 U[t] = ld3(velb, b0 + t * bS, P4, P8);
             }
@@ -569,7 +569,7 @@ __device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
                         const f3 q = grid_coord(X[t], origin, a.inv_h);
                         const Rng rg = point_range(q, G);
                         const bool inw = in_win(rg, gwin);
-                        if (inw && rng_full(rg))
+                        if (inw & rng_full(rg))
                             gv = gather_block(reinterpret_cast<const float4*>(win), gwin, rg, q);
                         else if (!rng_empty(rg))
                             gv = gather_any(reinterpret_cast<const float4*>(win), gwin, inw, a.gw.grid_f, rg, q, G);
