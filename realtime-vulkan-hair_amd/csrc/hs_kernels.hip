@@ -139,11 +139,12 @@ __device__ __forceinline__ float4 mesh_contact(const MeshArgs& M, float px, floa
     // candidates kMeshRound at a time: a candidate is one 48 B record holding
     // its key, bounding radius, id and vertices, so every load of a round is
     // in flight together and none waits on an id.  The bounding centre is
-    // formed from the vertices, as the host formed it for the radius.  The list is sorted by `key`
-    // (the distance from the cell's centre to the triangle's bounding sphere,
-    // rounded down); key - |x - centre| bounds the distance from below, so
-    // once that passes `reach` no later entry can win.  The nearest of equal
-    // distances is the lowest id (the order of the list does not matter).
+    // formed from the vertices, as the host formed it for the radius.  The
+    // list is sorted by `key` (the distance from the cell's centre to the
+    // triangle's bounding sphere, rounded down); key - |x - centre| bounds the
+    // distance from below, so once that passes `reach` no later entry can
+    // win.  The nearest of equal distances is the lowest id (the order of the
+    // list does not matter).
     bool more = true;
     for (uint32_t i = i0; i < i1 && more; i += kMeshRound) {
         float4 ra[kMeshRound], rb[kMeshRound], rc[kMeshRound];
@@ -200,8 +201,9 @@ __device__ __forceinline__ float4 mesh_contact(const MeshArgs& M, float px, floa
 
 // Mesh pass: the mesh contact of every stored point (the positions a step
 // starts from), one thread per point, into out[N][Sp].  Apart from the step
-// kernel, whose register budget holds it to two waves per SIMD, this runs at
-// full occupancy, which hides the scattered triangle loads.
+// kernel, whose register budget holds it to two or three waves per SIMD, this
+// runs at full occupancy (8), which hides the scattered record loads; what
+// bounds it is the bytes its lanes load, returned at a fixed rate per CU.
 __device__ __forceinline__ void mesh_point(const MeshArgs& M, const float* __restrict__ pos, uint32_t plane, float k,
                                            float4* __restrict__ out, uint32_t i)
 {
