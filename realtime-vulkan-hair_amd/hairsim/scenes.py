@@ -151,11 +151,14 @@ def head_mesh(subdiv=5, scale=(1.0, 1.2, 1.0)):
     return icosphere(subdiv, scale)
 
 
-def mesh_head_scene(name, seed, S, N, G, subdiv, scale=(1.0, 1.2, 1.0), margin=0.05, mesh_grid=(96, 114, 96),
+def mesh_head_scene(name, seed, S, N, G, subdiv, scale=(1.0, 1.2, 1.0), margin=0.05, mesh_grid=(176, 209, 176),
                     steps=120):
     """Roots on the scaled upper hemisphere (the head's scalp), strands along
     the surface normal; wind, noise, orbiting sphere and grid as plain-100k,
-    plus the head mesh collider."""
+    plus the head mesh collider.  The mesh grid only sets the contact search's
+    speed, not its result (at 1M x 64, 110 steps, the mesh pass measured 2.04 /
+    1.77 / 1.58 / 1.55 / 1.52 / 1.53 / 1.58 ms at 72, 96, 128, 144, 176, 192,
+    224 cells along x)."""
     rng = np.random.default_rng(seed)
     roots, nrm = hemisphere_roots(rng, S, scale=scale)
     L = rng.uniform(0.8, 1.2, S)
