@@ -744,8 +744,15 @@ __device__ __forceinline__ void step_body(const StepArgs& a, uint32_t wg)
     if (SPLAT) deferred_chunks(a, wg, win, bcast, reinterpret_cast<uint32_t*>(red), own_k, n_own);
 }
 
+// Wavefronts per SIMD the compiler budgets registers for: 4 for the 256 x 8
+// kernels with the mesh collider (they run on large meshes of many
+// workgroups, where four wavefronts outweigh the few registers spilled), the
+// occupancy the registers allow otherwise.
+template <int COL>
+constexpr int kStepWaves = (COL & kColMesh) && kW == 256 && kL == 8 ? 4 : 1;
+
 template <bool GATHER, bool SPLAT, int COL>
-__global__ __launch_bounds__(kW) void k_step(StepArgs a)
+__global__ __launch_bounds__(kW) __attribute__((amdgpu_waves_per_eu(kStepWaves<COL>))) void k_step(StepArgs a)
 {
     const uint32_t wg = a.swz ? xcd_swizzle(blockIdx.x, a.nwg) : blockIdx.x;
     if (wg >= a.nwg) return;
