@@ -313,7 +313,12 @@ int hs_grid_work(hs_ctx* ctx, uint32_t groom_id, hs_grid_work_stats* out);
 /* ------------------------------------------------------------------------ */
 
 typedef struct hs_mesh_desc {
-    uint32_t grid_dim[3];  /* triangle-grid cells per axis over the mesh bounds grown by margin */
+    /* triangle-grid cells per axis (each in [1, 1024]) over the mesh bounds
+     * grown by margin.  The grid sets only the contact search's speed and
+     * memory (48 B per listed cell-triangle pair), never its result; cells
+     * about a quarter of the margin searched fastest on a head-sized mesh of
+     * 20,480 triangles. */
+    uint32_t grid_dim[3];
     float margin;          /* contact range, > 0 */
 } hs_mesh_desc;
 
