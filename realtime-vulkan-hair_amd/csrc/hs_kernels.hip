@@ -442,9 +442,12 @@ __global__ __launch_bounds__(kTileThreads) void k_unpack(float4* __restrict__ ao
     const uint32_t ns = min(ts, S - p0);
     const uint32_t row = N + 1;
     const size_t plane = (size_t)N * Sp;
+    // ts (N + 1) <= kTileF4: indices and divisors stay below 2^16, where the
+    // reciprocal multiply divides exactly
+    const uint32_t m_ts = div_magic(ts), m_n = div_magic(N);
     for (uint32_t arr = 0; arr < 3; ++arr) {
         for (uint32_t e = threadIdx.x; e < ts * N; e += kTileThreads) {
-            const uint32_t pl = e % ts, i = e / ts;
+            const uint32_t i = div_small(e, ts, m_ts), pl = e - i * ts;
             if (pl >= ns) continue;
             const size_t o = (size_t)i * Sp + p0 + pl;
             float4 val = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -472,7 +475,7 @@ __global__ __launch_bounds__(kTileThreads) void k_unpack(float4* __restrict__ ao
         }
         __syncthreads();
         for (uint32_t e = threadIdx.x; e < ns * N; e += kTileThreads) {
-            const uint32_t pl = e / N, i = e % N;
+            const uint32_t pl = div_small(e, N, m_n), i = e - pl * N;
             aos[((size_t)perm[p0 + pl] * 3 + arr) * N + i] = tile[pl * row + i];
         }
         __syncthreads();
